@@ -1,0 +1,236 @@
+"""Shape-query benchmark on one MI355X.
+
+Workload (the full-zoo FP64 configuration): every shape of the zoo that has
+device kernels, each placed with a random rotation and a translation in
++-100 mm, queried with 2^22 seeded points and isotropic directions (SOA3D,
+float64).  One step = Contains and Inside on all points, DistanceToIn and
+SafetyToIn on the points outside the shape (step max 1e30), DistanceToOut and
+SafetyToOut on the points inside (local frame), for every shape.  Inputs are
+resident in HBM before the timed region starts.
+
+Prints one JSON line: throughput (Gqueries/s), the roofline of the dominant
+kernel (HBM bytes / launch duration from HIP events recorded on the launch
+stream inside the timed region), the plain-FP64-GPU denominator, and the
+oracle (C++ restatement of the reference Scalar path) timed on host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "shape Gqueries/s + nav steps/s vs plain FP64 GPU at 1e-9 mm; p99 basket latency"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per query: each input read once, each output written once.
+BYTES = {"contains": 25, "inside": 28, "dist_in": 64, "dist_out": 64, "safe_in": 32, "safe_out": 32}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--points", type=int, default=1 << 22)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+class Workload:
+    def __init__(self, n, device):
+        import torch
+
+        from vecgeom_amd import capi
+        from vecgeom_amd import geometry as G
+        from vecgeom_amd import queries as Q
+        from vecgeom_amd import workloads as W
+
+        self.Q = Q
+        rng = np.random.Generator(np.random.Philox(W.SEED_BASE + 2))
+        all_shapes = W.zoo_shapes()
+        self.shapes = [(nm, u) for nm, u in all_shapes if capi.lib().vg_shape_supported(u.type)]
+        self.missing = [nm for nm, u in all_shapes if not capi.lib().vg_shape_supported(u.type)]
+        xfs_all = [W.random_placement(rng) for _ in all_shapes]
+        self.xfs = [xf for (nm, u), xf in zip(all_shapes, xfs_all) if capi.lib().vg_shape_supported(u.type)]
+        self.geom, self.ids, _ = G.single_shape_geometry([u for _, u in self.shapes], self.xfs)
+        self.n = n
+        self.items = []
+        t = lambda a: tuple(torch.from_numpy(np.ascontiguousarray(a[i])).to(device) for i in range(3))
+        for k, ((name, u), xf, pid) in enumerate(zip(self.shapes, self.xfs, self.ids)):
+            inp = W.shape_inputs(self.geom.placed_extent(pid), xf, n, seed=W.SEED_BASE + 2 + 17 * k)
+            p, d, lp, ld = t(inp["p"]), t(inp["d"]), t(inp["lp"]), t(inp["ld"])
+            c = Q.contains(self.geom, pid, p).bool()
+            iin = torch.nonzero(c).flatten()
+            iout = torch.nonzero(~c).flatten()
+            pick = lambda s, idx: tuple(a[idx].contiguous() for a in s)
+            it = {
+                "name": name, "pid": pid, "p": p,
+                "pout": pick(p, iout), "dout": pick(d, iout), "smout": torch.full((iout.numel(),), 1e30,
+                                                                                   dtype=torch.float64, device=device),
+                "pin": pick(lp, iin), "din": pick(ld, iin), "smin": torch.full((iin.numel(),), 1e30,
+                                                                                dtype=torch.float64, device=device),
+                "host": {k2: inp[k2] for k2 in ("p", "d", "lp", "ld")}, "inside_idx": iin.cpu().numpy(),
+                "outside_idx": iout.cpu().numpy(),
+            }
+            n_in, n_out = iin.numel(), iout.numel()
+            it["out"] = {
+                "contains": torch.empty(n, dtype=torch.uint8, device=device),
+                "inside": torch.empty(n, dtype=torch.int32, device=device),
+                "dist_in": torch.empty(n_out, dtype=torch.float64, device=device),
+                "safe_in": torch.empty(n_out, dtype=torch.float64, device=device),
+                "dist_out": torch.empty(n_in, dtype=torch.float64, device=device),
+                "safe_out": torch.empty(n_in, dtype=torch.float64, device=device),
+            }
+            it["count"] = {"contains": n, "inside": n, "dist_in": n_out, "safe_in": n_out, "dist_out": n_in,
+                           "safe_out": n_in}
+            self.items.append(it)
+        self.queries_per_step = sum(sum(it["count"].values()) for it in self.items)
+
+    def launches(self):
+        """(key, callable) for every launch of one step."""
+        Q, g = self.Q, self.geom
+        out = []
+        for it in self.items:
+            pid, o = it["pid"], it["out"]
+            out += [
+                ((it["name"], "contains"), lambda it=it: Q.contains(g, it["pid"], it["p"], out=it["out"]["contains"])),
+                ((it["name"], "inside"), lambda it=it: Q.inside(g, it["pid"], it["p"], out=it["out"]["inside"])),
+                ((it["name"], "dist_in"), lambda it=it: Q.distance_to_in(g, it["pid"], it["pout"], it["dout"],
+                                                                         it["smout"], out=it["out"]["dist_in"])),
+                ((it["name"], "safe_in"), lambda it=it: Q.safety_to_in(g, it["pid"], it["pout"],
+                                                                       out=it["out"]["safe_in"])),
+                ((it["name"], "dist_out"), lambda it=it: Q.distance_to_out(g, it["pid"], it["pin"], it["din"],
+                                                                           it["smin"], out=it["out"]["dist_out"])),
+                ((it["name"], "safe_out"), lambda it=it: Q.safety_to_out(g, it["pid"], it["pin"],
+                                                                         out=it["out"]["safe_out"])),
+            ]
+        return out
+
+
+def cpu_baseline(wl, target_s):
+    """Oracle (C++ restatement of the reference Scalar kernels) on the host:
+    the same queries on a bounded, evenly strided sample of the same inputs."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    o = po.oracle()
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+    # Calibrate: one shape, all queries, small sample.
+    frac = 1.0 / 64
+    total_q = 0
+    t0 = time.perf_counter()
+    while True:
+        total_q = 0
+        t0 = time.perf_counter()
+        for it, (name, u), xf in zip(wl.items, wl.shapes, wl.xfs):
+            h = o.placed(u.type, u.params, xf.array())
+            hp = it["host"]
+            step = int(round(1 / frac))
+            sel_all = slice(0, wl.n, step)
+            iout = it["outside_idx"][::step]
+            iin = it["inside_idx"][::step]
+            P, D, LP, LD = hp["p"], hp["d"], hp["lp"], hp["ld"]
+            h.query(po.Q_CONTAINS, P[:, sel_all], nthreads=nthreads)
+            h.query(po.Q_INSIDE, P[:, sel_all], nthreads=nthreads)
+            h.query(po.Q_DIST_IN, P[:, iout], D[:, iout], np.full(len(iout), 1e30), nthreads=nthreads)
+            h.query(po.Q_SAFE_IN, P[:, iout], nthreads=nthreads)
+            h.query(po.Q_DIST_OUT, LP[:, iin], LD[:, iin], np.full(len(iin), 1e30), nthreads=nthreads)
+            h.query(po.Q_SAFE_OUT, LP[:, iin], nthreads=nthreads)
+            total_q += 2 * len(range(0, wl.n, step)) + 2 * len(iout) + 2 * len(iin)
+        dt = time.perf_counter() - t0
+        if dt >= 0.5 * target_s or frac >= 1.0:
+            break
+        frac = min(1.0, frac * max(2.0, target_s / max(dt, 1e-3)))
+    return {
+        "value": total_q / dt / 1e9,
+        "unit": "Gqueries/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": f"every {int(round(1 / frac))}th query of the same workload ({total_q} queries, {dt:.1f} s), "
+                  f"oracle C++ restatement of the reference Scalar backend, std::thread over contiguous ranges",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "described in DESIGN.md, not built)"}))
+        sys.exit(2)
+    import torch
+
+    from vecgeom_amd import queries as Q
+
+    Q.init(0)
+    dev = torch.device("cuda:0")
+    wl = Workload(args.points, dev)
+    launches = wl.launches()
+    stream = torch.cuda.current_stream()
+
+    def step():
+        for _, fn in launches:
+            fn()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # Timed region: K steps, with HIP events around each launch on the launch
+    # stream (torch's current stream, which the C-ABI calls use).
+    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in launches]
+          for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        for (key, fn), (a, b) in zip(launches, ev[s]):
+            a.record(stream)
+            fn()
+            b.record(stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = wl.queries_per_step * args.steps / elapsed / 1e9
+
+    per = {}
+    for s in range(args.steps):
+        for (key, fn), (a, b) in zip(launches, ev[s]):
+            per.setdefault(key, []).append(a.elapsed_time(b))
+    rows = []
+    for it in wl.items:
+        for q, cnt in it["count"].items():
+            ms = float(np.mean(per[(it["name"], q)]))
+            rows.append({"shape": it["name"], "query": q, "n": cnt, "avg_us": 1e3 * ms,
+                         "gqps": cnt / (ms * 1e-3) / 1e9 if ms > 0 else None,
+                         "gbs": cnt * BYTES[q] / (ms * 1e-3) / 1e9 if ms > 0 else None})
+    dom = max(rows, key=lambda r: r["avg_us"] * 1.0)
+    roof = {"bound": "hbm", "achieved": round(dom["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": f"ShapeQueryKernel<{dom['shape']},{dom['query']}>",
+            "bytes_per_query": BYTES[dom["query"]], "queries_per_launch": dom["n"]}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(wl, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gqueries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "shape zoo FP64, random placements, 2^%d points per shape, six queries"
+                               % int(np.log2(args.points)),
+                   "points_per_shape": args.points, "shapes": [nm for nm, _ in wl.shapes],
+                   "shapes_without_kernels": wl.missing, "queries_per_step": wl.queries_per_step},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "per_kernel": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()} for r in rows],
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,216 @@
+/*
+ * vecgeom_amd.h -- C-ABI of the MI355X (gfx950) shape-query and navigation
+ * library libvecgeom_amd.so.
+ *
+ * Every entry point takes plain pointers and sizes.  Array arguments of the
+ * query entry points are DEVICE pointers (hipMalloc'd or torch tensors on the
+ * GPU); `stream` is a hipStream_t passed as void* (NULL = default stream).
+ * Calls are asynchronous on `stream` unless stated otherwise.  Every entry
+ * point returns VG_OK (0) or a negative status; vg_last_error() gives the
+ * message for the calling thread.
+ *
+ * Frames follow the reference (volumes/PlacedVolume.h:144-250): Contains,
+ * Inside, DistanceToIn and SafetyToIn take points in the MOTHER frame of the
+ * placed volume (its placement transform is applied inside); DistanceToOut
+ * and SafetyToOut take points in the volume's LOCAL frame;
+ * PlacedDistanceToOut takes mother-frame points.
+ *
+ * Outputs follow the Scalar backend's virtual methods
+ * (volumes/ShapeImplementationHelper.h:135-318): a miss is +inf, Box
+ * DistanceToIn of an inside point is -1, Inside returns 0 (inside), 1
+ * (surface) or 2 (outside).
+ */
+#ifndef VECGEOM_AMD_H_
+#define VECGEOM_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+#define VG_OK 0
+#define VG_ERR_INVALID (-1)     /* bad argument, handle or id              */
+#define VG_ERR_HIP (-2)         /* a HIP runtime call failed               */
+#define VG_ERR_STATE (-3)       /* vg_init not called / device mismatch    */
+#define VG_ERR_UNSUPPORTED (-4) /* shape or feature not available          */
+#define VG_ERR_CAPACITY (-5)    /* request exceeds a configured limit      */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *vg_last_error(void);
+
+/* Library version string. */
+const char *vg_version(void);
+
+/*
+ * Selects the HIP device used by this process and prepares per-device
+ * state.  Replaces the device set-up of CudaManager
+ * (source/CudaManager.cpp:21-38).  Idempotent for the same device.
+ */
+int vg_init(int device);
+
+/* Releases per-device state; geometries must be freed first. */
+int vg_shutdown(void);
+
+/* ------------------------------------------------------------ shape types */
+/* Constructor parameters, in this order, for each type:
+ *   VG_BOX            dx dy dz                         (UnplacedBox.h:30)
+ *   VG_TUBE           rmin rmax dz sphi dphi [rad]     (UnplacedTube.h:68)
+ *   VG_ORB            r                                (UnplacedOrb.h:45)
+ *   VG_SPHERE         rmin rmax sphi dphi stheta dtheta [rad] (UnplacedSphere.h:219)
+ *   VG_CONE           rmin1 rmax1 rmin2 rmax2 dz sphi dphi [rad] (UnplacedCone.h:108)
+ *   VG_TRD            dx1 dx2 dy1 dy2 dz               (UnplacedTrd.h:73)
+ *   VG_TRAPEZOID      dz theta phi dy1 dx1 dx2 tanAlpha1 dy2 dx3 dx4 tanAlpha2
+ *                                                      (UnplacedTrapezoid.h:93)
+ *   VG_PARALLELEPIPED x y z alpha theta phi            (UnplacedParallelepiped.h:34)
+ *   VG_PARABOLOID     rlo rhi dz                       (UnplacedParaboloid.h:85)
+ *   VG_TORUS          rmin rmax rtor sphi dphi [rad]   (UnplacedTorus.h:61)
+ *   VG_POLYCONE       sphi dphi nz rmin[nz] rmax[nz] z[nz] (UnplacedPolycone.h:82)
+ *   VG_POLYHEDRON     phiStart phiDelta [deg] sides nz z[nz] rmin[nz] rmax[nz]
+ *                                                      (UnplacedPolyhedron.h:169)
+ */
+enum vg_shape_type {
+  VG_BOX = 1,
+  VG_TUBE = 2,
+  VG_ORB = 3,
+  VG_SPHERE = 4,
+  VG_CONE = 5,
+  VG_TRD = 6,
+  VG_TRAPEZOID = 7,
+  VG_PARALLELEPIPED = 8,
+  VG_PARABOLOID = 9,
+  VG_TORUS = 10,
+  VG_POLYCONE = 11,
+  VG_POLYHEDRON = 12
+};
+
+/* Returns 1 if the library has device kernels for this shape type. */
+int vg_shape_supported(int32_t shape_type);
+
+/* ------------------------------------------------------ geometry upload */
+/*
+ * Flattened volume hierarchy.  Host pointers; the library copies what it
+ * needs during vg_geometry_upload.  This replaces the per-object device
+ * construction of CudaManager::LoadGeometry + Synchronize
+ * (management/CudaManager.h:97-160, source/CudaManager.cpp:39-322) and the
+ * VPlacedVolume/VUnplacedVolume/LogicalVolume/Transformation3D CopyToGpu
+ * methods (volumes/PlacedVolume.h:283-316, volumes/UnplacedVolume.h:53-78,
+ * source/Transformation3D.cpp:206-221, source/LogicalVolume.cpp:151-168).
+ *
+ * shapes:    shape_type[n_shapes]; constructor parameters of shape s are
+ *            params[shape_param_begin[s] .. shape_param_begin[s+1]).
+ * logical:   logical_shape[n_logical]; daughters of logical volume l are the
+ *            placed ids daughters[logical_daughter_begin[l] .. [l+1]) in
+ *            LogicalVolume::PlaceDaughter order (source/LogicalVolume.cpp:84-102).
+ * placed:    placed_logical[n_placed]; placed_transform[12*p .. 12*p+12) =
+ *            tx ty tz r0..r8 as in Transformation3D(tx,..,r8)
+ *            (base/Transformation3D.h:95-101; local = R^T (master - t)).
+ * world:     placed id of the world volume (GeoManager::SetWorld).
+ */
+typedef struct vg_flat_geometry {
+  int32_t n_shapes;
+  const int32_t *shape_type;
+  const int32_t *shape_param_begin; /* n_shapes + 1 */
+  const double *params;
+  int32_t n_logical;
+  const int32_t *logical_shape;
+  const int32_t *logical_daughter_begin; /* n_logical + 1 */
+  const int32_t *daughters;
+  int32_t n_placed;
+  const int32_t *placed_logical;
+  const double *placed_transform; /* 12 * n_placed */
+  int32_t world;
+} vg_flat_geometry;
+
+typedef struct vg_geometry vg_geometry;
+
+/* Validates, precomputes cached shape values and bounding boxes, and copies
+ * everything to device memory (synchronous). */
+int vg_geometry_upload(const vg_flat_geometry *geometry, vg_geometry **out);
+
+/* Frees device memory of a geometry (CudaManager::CleanGpu,
+ * source/CudaManager.cpp:173-192). */
+int vg_geometry_free(vg_geometry *geometry);
+
+/* Maximum depth of the placed-volume tree (GeoManager::getMaxDepth,
+ * management/GeoManager.h:129-200); the world is depth 1. */
+int vg_geometry_max_depth(const vg_geometry *geometry, int32_t *depth);
+
+/* Local-frame extent of the shape of placed volume `placed`
+ * (VPlacedVolume::Extent -> UnplacedX::Extent, volumes/PlacedVolume.h:260).
+ * lo[3], hi[3] host pointers. */
+int vg_geometry_placed_extent(const vg_geometry *geometry, int32_t placed, double *lo, double *hi);
+
+/* Axis-aligned bounding box of placed volume `placed` in its mother frame,
+ * padded as ABBoxManager::ComputeABBox does (source/ABBoxManager.cpp:128-161).
+ * lo[3], hi[3] host pointers. */
+int vg_geometry_placed_aabb(const vg_geometry *geometry, int32_t placed, double *lo, double *hi);
+
+/* --------------------------------------------------- basket shape queries */
+/* SOA3D basket forms of the VPlacedVolume virtuals
+ * (volumes/PlacedVolume.h:182-250; loops in
+ * volumes/ShapeImplementationHelper.h:588-748).  `placed` is a placed-volume
+ * id of the uploaded geometry.  x,y,z (and dx,dy,dz) are device arrays of
+ * length n; n may be any value >= 0.  step_max may be NULL (= +inf, the
+ * reference's default argument). */
+
+/* VPlacedVolume::Contains(SOA3D, bool*)  (volumes/PlacedVolume.h:147) */
+int vg_contains(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z, size_t n,
+                uint8_t *out, void *stream);
+
+/* VPlacedVolume::Inside(SOA3D, Inside_t*)  (volumes/PlacedVolume.h:165) */
+int vg_inside(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z, size_t n,
+              int32_t *out, void *stream);
+
+/* VPlacedVolume::DistanceToIn(SOA3D, SOA3D, step_max*, out*)  (volumes/PlacedVolume.h:176) */
+int vg_distance_to_in(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z,
+                      const double *dx, const double *dy, const double *dz, const double *step_max, size_t n,
+                      double *out, void *stream);
+
+/* VPlacedVolume::DistanceToInMinimize  (volumes/PlacedVolume.h:181; semantics
+ * ShapeImplementationHelper.h:637-658): result = DistanceToIn(p, d,
+ * inout_dist[i]); if result < inout_dist[i] then inout_dist[i] = result and
+ * inout_next[i] = daughter_index. */
+int vg_distance_to_in_minimize(const vg_geometry *g, int32_t placed, int32_t daughter_index, const double *x,
+                               const double *y, const double *z, const double *dx, const double *dy, const double *dz,
+                               size_t n, double *inout_dist, int32_t *inout_next, void *stream);
+
+/* VPlacedVolume::DistanceToOut(SOA3D, SOA3D, step_max*, out* [, next*])
+ * (volumes/PlacedVolume.h:212-221).  Local-frame points.  When next_node is not
+ * NULL it receives -1 where out[i] < step_max[i] and -2 otherwise
+ * (ShapeImplementationHelper.h:677-692). */
+int vg_distance_to_out(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z,
+                       const double *dx, const double *dy, const double *dz, const double *step_max, size_t n,
+                       double *out, int32_t *next_node, void *stream);
+
+/* VPlacedVolume::PlacedDistanceToOut (volumes/PlacedVolume.h:207) in basket form:
+ * mother-frame points, transformed inside. */
+int vg_placed_distance_to_out(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z,
+                              const double *dx, const double *dy, const double *dz, const double *step_max, size_t n,
+                              double *out, void *stream);
+
+/* VPlacedVolume::SafetyToIn(SOA3D, out*)  (volumes/PlacedVolume.h:231) */
+int vg_safety_to_in(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z, size_t n,
+                    double *out, void *stream);
+
+/* VPlacedVolume::SafetyToInMinimize  (volumes/PlacedVolume.h:237; semantics
+ * ShapeImplementationHelper.h:707-720): inout[i] = min(inout[i], SafetyToIn). */
+int vg_safety_to_in_minimize(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z,
+                             size_t n, double *inout, void *stream);
+
+/* VPlacedVolume::SafetyToOut(SOA3D, out*)  (volumes/PlacedVolume.h:243); local frame. */
+int vg_safety_to_out(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z, size_t n,
+                     double *out, void *stream);
+
+/* VPlacedVolume::SafetyToOutMinimize  (volumes/PlacedVolume.h:249; semantics
+ * ShapeImplementationHelper.h:733-746). */
+int vg_safety_to_out_minimize(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z,
+                              size_t n, double *inout, void *stream);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* VECGEOM_AMD_H_ */

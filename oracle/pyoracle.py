@@ -1,0 +1,123 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes access to the CPU restatement (``oracle/lib/liboracle*.so``) and to the
+shim over the real reference build (``oracle/_ref/libvgref*.so``).  Both
+export the same entry points (prefix ``vgo_`` / ``vgref_``).  Only tests/,
+``__graft_entry__.smoke()`` and the cpu_baseline leg of bench.py import this
+module; the product package never does.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+Q_CONTAINS, Q_INSIDE, Q_DIST_IN, Q_DIST_OUT, Q_SAFE_IN, Q_SAFE_OUT, Q_PLACED_DIST_OUT = range(7)
+QUERY_DTYPE = {
+    Q_CONTAINS: np.uint8,
+    Q_INSIDE: np.int32,
+    Q_DIST_IN: np.float64,
+    Q_DIST_OUT: np.float64,
+    Q_SAFE_IN: np.float64,
+    Q_SAFE_OUT: np.float64,
+    Q_PLACED_DIST_OUT: np.float64,
+}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Lib:
+    """One loaded oracle flavour (restatement or reference shim)."""
+
+    def __init__(self, path, prefix):
+        self.path = path
+        self.prefix = prefix
+        self.lib = ctypes.CDLL(path)
+        f = self._f
+        f("placed_create").restype = ctypes.c_void_p
+        f("placed_create").argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        f("placed_destroy").argtypes = [ctypes.c_void_p]
+        f("query").restype = ctypes.c_int
+        f("query").argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_long] + [ctypes.c_void_p] * 8 + [ctypes.c_int]
+        f("shape_extent").argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+
+    def _f(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    def has(self, name):
+        return hasattr(self.lib, self.prefix + name)
+
+    def placed(self, shape_type, params, xf12):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        x = np.ascontiguousarray(xf12, dtype=np.float64)
+        h = self._f("placed_create")(int(shape_type), _ptr(p), len(p), _ptr(x))
+        if not h:
+            raise ValueError(f"{self.prefix}placed_create refused shape {shape_type} params {list(p)}")
+        return Placed(self, h)
+
+
+class Placed:
+    def __init__(self, lib, handle):
+        self.lib = lib
+        self.h = handle
+
+    def extent(self):
+        lo = np.zeros(3)
+        hi = np.zeros(3)
+        self.lib._f("shape_extent")(self.h, _ptr(lo), _ptr(hi))
+        return lo, hi
+
+    def query(self, q, p, d=None, step_max=None, nthreads=1):
+        """p, d: (3, n) float64 arrays (SOA).  Returns the output array."""
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        n = p.shape[1]
+        out = np.empty(n, dtype=QUERY_DTYPE[q])
+        if d is not None:
+            d = np.ascontiguousarray(d, dtype=np.float64)
+            dx, dy, dz = _ptr(d[0]), _ptr(d[1]), _ptr(d[2])
+        else:
+            dx = dy = dz = None
+        sm = None if step_max is None else np.ascontiguousarray(step_max, dtype=np.float64)
+        rc = self.lib._f("query")(self.h, q, n, _ptr(p[0]), _ptr(p[1]), _ptr(p[2]), dx, dy, dz, _ptr(sm), _ptr(out),
+                                  int(nthreads))
+        if rc != 0:
+            raise RuntimeError(f"{self.lib.prefix}query failed rc={rc}")
+        return out
+
+    def __del__(self):
+        try:
+            self.lib._f("placed_destroy")(self.h)
+        except Exception:
+            pass
+
+
+_cache = {}
+
+
+def oracle(fast=False):
+    """The restatement (strict by default)."""
+    key = ("vgo", fast)
+    if key not in _cache:
+        name = "liboracle_fast.so" if fast else "liboracle.so"
+        path = os.path.join(HERE, "lib", name)
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run __graft_entry__.build()")
+        _cache[key] = Lib(path, "vgo_")
+    return _cache[key]
+
+
+def reference(fast=False):
+    """The real reference build, or None where it was not built."""
+    key = ("vgref", fast)
+    if key not in _cache:
+        name = "libvgref_fast.so" if fast else "libvgref.so"
+        path = os.path.join(HERE, "_ref", name)
+        _cache[key] = Lib(path, "vgref_") if os.path.exists(path) else None
+    return _cache[key]

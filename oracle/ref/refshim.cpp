@@ -1,0 +1,152 @@
+// ORACLE -- test infrastructure only.
+//
+// C-ABI shim over the real reference build (compiled from the reference's own
+// sources by oracle/Makefile into oracle/_ref/libvgref.so).  It exports the
+// same entry points as the oracle restatement (vgo_capi.cpp) with the prefix
+// vgref_, so tests check the restatement against the reference and write
+// golden vectors from it.  This file contains no reference code: it only
+// constructs reference objects and calls their public virtual interface
+// (volumes/PlacedVolume.h:144-250).
+#include <cstdint>
+#include <cstring>
+#include <cmath>
+#include <vector>
+
+#include "base/Global.h"
+#include "base/Transformation3D.h"
+#include "base/Vector3D.h"
+#include "volumes/LogicalVolume.h"
+#include "volumes/PlacedVolume.h"
+#include "volumes/Box.h"
+#include "volumes/Tube.h"
+#include "volumes/Orb.h"
+#include "volumes/Sphere.h"
+#include "volumes/Cone.h"
+#include "volumes/Trd.h"
+#include "volumes/Trapezoid.h"
+#include "volumes/Parallelepiped.h"
+#include "volumes/Paraboloid.h"
+#include "volumes/Torus.h"
+#include "volumes/Polycone.h"
+#include "volumes/Polyhedron.h"
+
+using namespace vecgeom;
+
+namespace {
+
+struct RefPlaced {
+  VUnplacedVolume *unplaced = nullptr;
+  LogicalVolume *logical = nullptr;
+  Transformation3D *xf = nullptr;
+  VPlacedVolume const *placed = nullptr;
+  int type = 0;
+  std::vector<double> params;
+};
+
+// The reference has no Extent for the parallelepiped
+// (volumes/PlacedVolume.h:260-263 asserts); this bound of its 8 corners is
+// the convention shared with the oracle and the product.
+void ParallelepipedExtent(const std::vector<double> &p, double *lo, double *hi) {
+  const double ta = std::tan(p[3]), tt = std::tan(p[4]);
+  const double ex = p[0] + std::fabs(p[1] * ta) + std::fabs(p[2] * tt * std::cos(p[5]));
+  const double ey = p[1] + std::fabs(p[2] * tt * std::sin(p[5]));
+  const double ez = p[2];
+  lo[0] = -ex; lo[1] = -ey; lo[2] = -ez;
+  hi[0] = ex; hi[1] = ey; hi[2] = ez;
+}
+
+VUnplacedVolume *MakeUnplaced(int type, const double *p, int n) {
+  switch (type) {
+    case 1: if (n == 3) return new UnplacedBox(p[0], p[1], p[2]); break;
+    case 2: if (n == 5) return new UnplacedTube(p[0], p[1], p[2], p[3], p[4]); break;
+    case 3: if (n == 1) return new UnplacedOrb(p[0]); break;
+    case 4: if (n == 6) return new UnplacedSphere(p[0], p[1], p[2], p[3], p[4], p[5]); break;
+    case 5: if (n == 7) return new UnplacedCone(p[0], p[1], p[2], p[3], p[4], p[5], p[6]); break;
+    case 6: if (n == 5) return new UnplacedTrd(p[0], p[1], p[2], p[3], p[4]); break;
+    case 7: if (n == 11) return new UnplacedTrapezoid(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9], p[10]); break;
+    case 8: if (n == 6) return new UnplacedParallelepiped(p[0], p[1], p[2], p[3], p[4], p[5]); break;
+    case 9: if (n == 3) return new UnplacedParaboloid(p[0], p[1], p[2]); break;
+    case 10: if (n == 5) return new UnplacedTorus(p[0], p[1], p[2], p[3], p[4]); break;
+    case 11: {
+      // sphi, dphi, nz, rmin[nz], rmax[nz], z[nz]
+      if (n < 3) break;
+      int nz = (int)p[2];
+      if (n != 3 + 3 * nz) break;
+      std::vector<double> rmin(p + 3, p + 3 + nz), rmax(p + 3 + nz, p + 3 + 2 * nz), z(p + 3 + 2 * nz, p + 3 + 3 * nz);
+      return new UnplacedPolycone(p[0], p[1], nz, rmin.data(), rmax.data(), z.data());
+    }
+    case 12: {
+      // phiStart(deg), phiDelta(deg), sides, nz, z[nz], rmin[nz], rmax[nz]
+      if (n < 4) break;
+      int nz = (int)p[3];
+      if (n != 4 + 3 * nz) break;
+      std::vector<double> z(p + 4, p + 4 + nz), rmin(p + 4 + nz, p + 4 + 2 * nz), rmax(p + 4 + 2 * nz, p + 4 + 3 * nz);
+      return new UnplacedPolyhedron(p[0], p[1], (int)p[2], nz, z.data(), rmin.data(), rmax.data());
+    }
+  }
+  return nullptr;
+}
+
+Vector3D<Precision> V(const double *x, const double *y, const double *z, long i) {
+  return Vector3D<Precision>(x[i], y[i], z[i]);
+}
+
+}  // namespace
+
+extern "C" {
+
+void *vgref_placed_create(int type, const double *params, int nparams, const double *xf12) {
+  VUnplacedVolume *u = MakeUnplaced(type, params, nparams);
+  if (!u) return nullptr;
+  auto *r = new RefPlaced;
+  r->unplaced = u;
+  r->logical = new LogicalVolume("shape", u);
+  r->xf = new Transformation3D(xf12[0], xf12[1], xf12[2], xf12[3], xf12[4], xf12[5], xf12[6], xf12[7], xf12[8],
+                               xf12[9], xf12[10], xf12[11]);
+  r->placed = r->logical->Place(r->xf);
+  r->type = type;
+  r->params.assign(params, params + nparams);
+  return r;
+}
+
+void vgref_placed_destroy(void *) {
+  // Reference objects register themselves in GeoManager; they are kept
+  // alive for the life of the process.
+}
+
+int vgref_query(void *h, int query, long n, const double *px, const double *py, const double *pz, const double *dx,
+                const double *dy, const double *dz, const double *step_max, void *out, int) {
+  auto *r = static_cast<RefPlaced *>(h);
+  if (!r) return -1;
+  VPlacedVolume const *pv = r->placed;
+  for (long i = 0; i < n; ++i) {
+    double sm = step_max ? step_max[i] : kInfinity;
+    switch (query) {
+      case 0: static_cast<uint8_t *>(out)[i] = pv->Contains(V(px, py, pz, i)) ? 1 : 0; break;
+      case 1: static_cast<int32_t *>(out)[i] = (int32_t)pv->Inside(V(px, py, pz, i)); break;
+      case 2: static_cast<double *>(out)[i] = pv->DistanceToIn(V(px, py, pz, i), V(dx, dy, dz, i), sm); break;
+      case 3: static_cast<double *>(out)[i] = pv->DistanceToOut(V(px, py, pz, i), V(dx, dy, dz, i), sm); break;
+      case 4: static_cast<double *>(out)[i] = pv->SafetyToIn(V(px, py, pz, i)); break;
+      case 5: static_cast<double *>(out)[i] = pv->SafetyToOut(V(px, py, pz, i)); break;
+      case 6: static_cast<double *>(out)[i] = pv->PlacedDistanceToOut(V(px, py, pz, i), V(dx, dy, dz, i), sm); break;
+      default: return -2;
+    }
+  }
+  return 0;
+}
+
+void vgref_shape_extent(void *h, double *lo, double *hi) {
+  auto *r = static_cast<RefPlaced *>(h);
+  if (r->type == 8) {
+    ParallelepipedExtent(r->params, lo, hi);
+    return;
+  }
+  Vector3D<Precision> l, u;
+  r->placed->Extent(l, u);
+  for (int i = 0; i < 3; ++i) {
+    lo[i] = l[i];
+    hi[i] = u[i];
+  }
+}
+
+}  // extern "C"

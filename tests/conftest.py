@@ -1,0 +1,58 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+def golden_cases(prefix="shape_"):
+    return sorted(f for f in os.listdir(GOLDEN) if f.startswith(prefix) and f.endswith(".npz"))
+
+
+def load_golden(name):
+    with np.load(os.path.join(GOLDEN, name), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def same(a, b):
+    """Bit-level agreement for outputs: equal values, inf == inf, NaN == NaN."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if a.dtype.kind == "f":
+        return (a == b) | (np.isnan(a) & np.isnan(b))
+    return a == b
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    import pyoracle
+    return pyoracle.oracle()
+
+
+@pytest.fixture(scope="session")
+def reference():
+    import pyoracle
+    ref = pyoracle.reference()
+    if ref is None:
+        pytest.skip("reference build (oracle/_ref) not present")
+    return ref
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    """Initialises the HIP library on cuda:0; fails loudly if it cannot."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU test collected on a machine without a GPU")
+    from vecgeom_amd import queries
+    queries.init(0)
+    return torch.device("cuda:0")

@@ -1,0 +1,79 @@
+"""The CPU restatement (oracle) against the reference's own outputs.
+
+Golden vectors come from the real reference build (tests/golden/
+make_golden.py); where the reference build is present, the oracle is also
+compared with it on larger random baskets.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from conftest import golden_cases, load_golden, same
+
+QMAP = {
+    "contains": po.Q_CONTAINS, "inside": po.Q_INSIDE, "dist_in": po.Q_DIST_IN, "dist_out": po.Q_DIST_OUT,
+    "safe_in": po.Q_SAFE_IN, "safe_out": po.Q_SAFE_OUT, "placed_dist_out": po.Q_PLACED_DIST_OUT,
+}
+# Torus DistanceToIn under a rotated placement: the reference transforms the
+# direction with the point transform, translation included
+# (volumes/kernel/TorusImplementation.h:906); the oracle and the product use
+# TransformDirection.  Parity for it is checked on the identity placement.
+KNOWN_DEVIATIONS = {("Torus", "rot", "dist_in")}
+
+
+def run_query(placed, qn, g):
+    if qn in ("dist_out", "safe_out"):
+        return placed.query(QMAP[qn], g["lp"], g["ld"] if qn == "dist_out" else None,
+                            g["sm"] if qn == "dist_out" else None)
+    d = g["d"] if qn in ("dist_in", "placed_dist_out") else None
+    sm = g["sm"] if qn in ("dist_in", "placed_dist_out") else None
+    return placed.query(QMAP[qn], g["p"], d, sm)
+
+
+def supported(oracle, g):
+    try:
+        oracle.placed(int(g["type"]), g["params"], g["xf"])
+        return True
+    except ValueError:
+        return False
+
+
+@pytest.mark.parametrize("case", golden_cases())
+def test_oracle_matches_reference_golden(oracle, case):
+    g = load_golden(case)
+    if not supported(oracle, g):
+        pytest.skip("shape not yet restated in the oracle")
+    shape = case.split("_")[1].replace("C2", "")
+    kind = case.split("_")[2][:-4]
+    h = oracle.placed(int(g["type"]), g["params"], g["xf"])
+    for qn in QMAP:
+        if (shape, kind, qn) in KNOWN_DEVIATIONS:
+            continue
+        got = run_query(h, qn, g)
+        ok = same(got, g[qn])
+        assert ok.all(), f"{case} {qn}: {int((~ok).sum())} lanes differ, first at {np.flatnonzero(~ok)[:5]}"
+
+
+@pytest.mark.parametrize("case", golden_cases())
+def test_oracle_matches_reference_random(oracle, reference, case):
+    """Larger random baskets against the live reference build."""
+    g = load_golden(case)
+    if not supported(oracle, g):
+        pytest.skip("shape not yet restated in the oracle")
+    from vecgeom_amd import workloads as W
+    from vecgeom_amd import geometry as G
+    shape = case.split("_")[1].replace("C2", "")
+    kind = case.split("_")[2][:-4]
+    xf = G.Transformation3D.from_array(g["xf"])
+    o = oracle.placed(int(g["type"]), g["params"], g["xf"])
+    r = reference.placed(int(g["type"]), g["params"], g["xf"])
+    n = 20000
+    inp = W.shape_inputs(o.extent(), xf, n, seed=12345 + len(case))
+    gg = {"p": inp["p"], "d": inp["d"], "lp": inp["lp"], "ld": inp["ld"], "sm": np.full(n, 1e30)}
+    for qn in QMAP:
+        if (shape, kind, qn) in KNOWN_DEVIATIONS:
+            continue
+        a = run_query(o, qn, gg)
+        b = run_query(r, qn, gg)
+        ok = same(a, b)
+        assert ok.all(), f"{case} {qn}: {int((~ok).sum())} of {n} lanes differ"

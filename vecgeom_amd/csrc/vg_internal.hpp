@@ -1,0 +1,107 @@
+// Host-side internals of libvecgeom_amd: the uploaded geometry and the
+// dispatch tables shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/vecgeom_amd.h"
+#include "vg_math.hpp"
+
+namespace vga {
+
+// Device record of one placed volume (also kept on the host).
+struct PlacedRec {
+  int32_t logical;
+  int32_t shape;      // index into shape tables
+  int32_t type;       // vg_shape_type
+  int32_t xfkind;     // XfKind of its placement
+  int32_t param_off;  // offset (doubles) of the shape's parameter block
+  int32_t dbegin;     // daughters of its logical volume: [dbegin, dend)
+  int32_t dend;
+  int32_t pad;
+  Xf xf;
+};
+static_assert(sizeof(PlacedRec) % 8 == 0, "PlacedRec must be 8-byte sized");
+
+struct ShapeRec {
+  int32_t type;
+  int32_t param_off;
+  int32_t param_len;
+  int32_t pad;
+};
+
+// Upper bound on the size of one shape parameter block (doubles).
+constexpr int kMaxParamDoubles = 4096;
+
+}  // namespace vga
+
+struct vg_geometry {
+  int device = -1;
+  int max_depth = 0;
+  int32_t world = -1;
+  std::vector<vga::ShapeRec> shapes;
+  std::vector<double> params;  // concatenated parameter blocks
+  std::vector<vga::PlacedRec> placed;
+  std::vector<int32_t> logical_shape;
+  std::vector<int32_t> logical_dbegin;  // n_logical + 1
+  std::vector<int32_t> daughters;
+  std::vector<double> aabb;             // 6 per placed: lo xyz, hi xyz (mother frame)
+  // Device copies.
+  double *d_params = nullptr;
+  vga::PlacedRec *d_placed = nullptr;
+  int32_t *d_daughters = nullptr;
+  float *d_aabb = nullptr;  // 6 floats per placed (ABBoxManager keeps floats)
+};
+
+namespace vga {
+
+// Error handling (vg_capi.cpp).
+int SetError(int code, const std::string &msg);
+int CheckHip(hipError_t e, const char *what);
+int EnsureDevice();
+
+// Builds the parameter block of one shape from its constructor parameters.
+// Returns an empty vector and sets *err on invalid input.
+std::vector<double> BuildParams(int type, const double *ctor, int n, std::string *err);
+// Local-frame axis-aligned extent of a shape (UnplacedX::Extent).
+bool ShapeExtent(int type, const double *block, double lo[3], double hi[3]);
+
+// Query kinds dispatched by vg_shape_kernels.hip.
+enum Query : int {
+  kQContains = 0,
+  kQInside,
+  kQDistIn,
+  kQDistInMin,
+  kQDistOut,
+  kQPlacedDistOut,
+  kQSafeIn,
+  kQSafeInMin,
+  kQSafeOut,
+  kQSafeOutMin,
+};
+
+struct QArgs {
+  const double *x, *y, *z;
+  const double *dx, *dy, *dz;
+  const double *sm;  // step max (nullable)
+  void *out;         // uint8 / int32 / double, or inout distances
+  int32_t *next;     // node index / next daughter (nullable)
+  int32_t did;       // daughter index for DistanceToInMinimize
+  int32_t pad;
+  uint64_t n;
+};
+
+// Launches one basket query on placed volume `placed` (vg_shape_kernels.hip).
+int LaunchQuery(const vg_geometry *g, int32_t placed, int q, const QArgs &a, hipStream_t s);
+bool ShapeHasKernels(int type);
+
+}  // namespace vga
+
+namespace vga {
+// Zoo shapes beyond Box and Tube (vg_params_zoo.cpp).
+std::vector<double> BuildZooParams(int type, const double *ctor, int n, std::string *err);
+bool ZooExtent(int type, const double *block, double lo[3], double hi[3]);
+}  // namespace vga

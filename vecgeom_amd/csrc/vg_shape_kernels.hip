@@ -1,0 +1,243 @@
+// Basket shape-query kernels for gfx950.
+//
+// One launch serves one placed volume and one query over an SOA3D basket
+// (the reference's ShapeImplementationHelper basket loops,
+// volumes/ShapeImplementationHelper.h:588-748, and the per-point CUDA
+// kernels of source/benchmarking/Benchmarker.cu:13-79).  Design:
+//  * the shape parameter block and the placement transform are kernel
+//    arguments passed by value, so every lane reads them from SGPRs: no
+//    pointer chase per point (the reference does a device virtual call);
+//  * the transform kind is a template parameter, as Transform<t,r> is in the
+//    reference, so identity placements do no arithmetic;
+//  * each lane handles IT consecutive points with 16-byte loads and stores
+//    from each SOA array (global_load_dwordx4), keeping enough bytes in
+//    flight per CU to run at HBM speed;
+//  * 256-thread blocks (4 wave64), grid = ceil(n / (256*IT)).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "shapes.hpp"
+#include "vg_internal.hpp"
+
+namespace vga {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+template <int IT>
+struct alignas(8 * IT) DVec {
+  double v[IT];
+};
+template <int IT>
+struct alignas(4 * IT) IVec {
+  int32_t v[IT];
+};
+template <int IT>
+struct alignas(IT) BVec {
+  uint8_t v[IT];
+};
+
+template <int IT>
+__device__ __forceinline__ void LoadD(const double *__restrict__ p, double (&o)[IT]) {
+  DVec<IT> t = *reinterpret_cast<const DVec<IT> *>(p);
+#pragma unroll
+  for (int k = 0; k < IT; ++k) o[k] = t.v[k];
+}
+
+template <class S, int XK, int Q>
+__device__ __forceinline__ void Eval(const typename S::P &P, const Xf &X, V3 p, V3 d, double sm, double &rd,
+                                     int32_t &ri) {
+  if constexpr (Q == kQContains) {
+    ri = S::Contains(P, XfPoint<XK>(X, p)) ? 1 : 0;
+  } else if constexpr (Q == kQInside) {
+    ri = S::Inside(P, XfPoint<XK>(X, p));
+  } else if constexpr (Q == kQDistIn || Q == kQDistInMin) {
+    rd = S::DistanceToIn(P, XfPoint<XK>(X, p), XfDir<XK>(X, d), sm);
+  } else if constexpr (Q == kQDistOut) {
+    rd = S::DistanceToOut(P, p, d, sm);
+  } else if constexpr (Q == kQPlacedDistOut) {
+    rd = S::DistanceToOut(P, XfPoint<XK>(X, p), XfDir<XK>(X, d), sm);
+  } else if constexpr (Q == kQSafeIn || Q == kQSafeInMin) {
+    rd = S::SafetyToIn(P, XfPoint<XK>(X, p));
+  } else {
+    rd = S::SafetyToOut(P, p);
+  }
+}
+
+constexpr bool NeedsDir(int q) { return q == kQDistIn || q == kQDistInMin || q == kQDistOut || q == kQPlacedDistOut; }
+constexpr bool NeedsStep(int q) { return q == kQDistIn || q == kQDistOut || q == kQPlacedDistOut; }
+
+template <class S, int XK, int Q, int IT>
+__global__ __launch_bounds__(kBlock) void ShapeQueryKernel(const typename S::P P, const Xf X, const QArgs a) {
+  const uint64_t i0 = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * IT;
+  if (i0 >= a.n) return;
+  double px[IT], py[IT], pz[IT], dx[IT], dy[IT], dz[IT], sm[IT];
+  const bool full = i0 + IT <= a.n;
+  if (full) {
+    LoadD<IT>(a.x + i0, px);
+    LoadD<IT>(a.y + i0, py);
+    LoadD<IT>(a.z + i0, pz);
+    if constexpr (NeedsDir(Q)) {
+      LoadD<IT>(a.dx + i0, dx);
+      LoadD<IT>(a.dy + i0, dy);
+      LoadD<IT>(a.dz + i0, dz);
+    }
+    if constexpr (NeedsStep(Q)) {
+      if (a.sm) {
+        LoadD<IT>(a.sm + i0, sm);
+      } else {
+#pragma unroll
+        for (int k = 0; k < IT; ++k) sm[k] = kInfinity;
+      }
+    }
+    if constexpr (Q == kQDistInMin || Q == kQSafeInMin || Q == kQSafeOutMin) {
+      // Minimize variants read the running minimum in place.
+      LoadD<IT>(reinterpret_cast<const double *>(a.out) + i0, sm);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      const uint64_t i = i0 + k < a.n ? i0 + k : i0;
+      px[k] = a.x[i];
+      py[k] = a.y[i];
+      pz[k] = a.z[i];
+      if constexpr (NeedsDir(Q)) {
+        dx[k] = a.dx[i];
+        dy[k] = a.dy[i];
+        dz[k] = a.dz[i];
+      }
+      if constexpr (NeedsStep(Q)) sm[k] = a.sm ? a.sm[i] : kInfinity;
+      if constexpr (Q == kQDistInMin || Q == kQSafeInMin || Q == kQSafeOutMin)
+        sm[k] = reinterpret_cast<const double *>(a.out)[i];
+    }
+  }
+
+  double rd[IT];
+  int32_t ri[IT];
+#pragma unroll
+  for (int k = 0; k < IT; ++k) {
+    V3 p{px[k], py[k], pz[k]};
+    V3 d{0, 0, 0};
+    if constexpr (NeedsDir(Q)) d = V3{dx[k], dy[k], dz[k]};
+    double s = 0;
+    if constexpr (NeedsStep(Q) || Q == kQDistInMin) s = sm[k];
+    Eval<S, XK, Q>(P, X, p, d, s, rd[k], ri[k]);
+  }
+
+  if constexpr (Q == kQContains) {
+    uint8_t *o = reinterpret_cast<uint8_t *>(a.out) + i0;
+    if (full) {
+      BVec<IT> t;
+#pragma unroll
+      for (int k = 0; k < IT; ++k) t.v[k] = (uint8_t)ri[k];
+      *reinterpret_cast<BVec<IT> *>(o) = t;
+    } else {
+      for (int k = 0; i0 + k < a.n; ++k) o[k] = (uint8_t)ri[k];
+    }
+  } else if constexpr (Q == kQInside) {
+    int32_t *o = reinterpret_cast<int32_t *>(a.out) + i0;
+    if (full) {
+      IVec<IT> t;
+#pragma unroll
+      for (int k = 0; k < IT; ++k) t.v[k] = ri[k];
+      *reinterpret_cast<IVec<IT> *>(o) = t;
+    } else {
+      for (int k = 0; i0 + k < a.n; ++k) o[k] = ri[k];
+    }
+  } else if constexpr (Q == kQDistInMin) {
+    // ShapeImplementationHelper.h:637-658
+    double *o = reinterpret_cast<double *>(a.out) + i0;
+    for (int k = 0; k < IT && i0 + k < a.n; ++k) {
+      if (rd[k] < sm[k]) {
+        o[k] = rd[k];
+        a.next[i0 + k] = a.did;
+      }
+    }
+  } else if constexpr (Q == kQSafeInMin || Q == kQSafeOutMin) {
+    // ShapeImplementationHelper.h:707-746
+    double *o = reinterpret_cast<double *>(a.out) + i0;
+    for (int k = 0; k < IT && i0 + k < a.n; ++k) o[k] = (rd[k] < sm[k]) ? rd[k] : sm[k];
+  } else {
+    double *o = reinterpret_cast<double *>(a.out) + i0;
+    if (full) {
+      DVec<IT> t;
+#pragma unroll
+      for (int k = 0; k < IT; ++k) t.v[k] = rd[k];
+      *reinterpret_cast<DVec<IT> *>(o) = t;
+    } else {
+      for (int k = 0; i0 + k < a.n; ++k) o[k] = rd[k];
+    }
+    if constexpr (Q == kQDistOut) {
+      // ShapeImplementationHelper.h:677-692
+      if (a.next) {
+        for (int k = 0; k < IT && i0 + k < a.n; ++k) a.next[i0 + k] = (rd[k] < sm[k]) ? -1 : -2;
+      }
+    }
+  }
+}
+
+bool Aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+template <class S, int XK, int Q>
+int LaunchQ(const typename S::P &P, const Xf &X, const QArgs &a, hipStream_t s) {
+  if (a.n == 0) return VG_OK;
+  bool al = Aligned16(a.x) && Aligned16(a.y) && Aligned16(a.z) && Aligned16(a.out);
+  if (NeedsDir(Q)) al = al && Aligned16(a.dx) && Aligned16(a.dy) && Aligned16(a.dz);
+  if (NeedsStep(Q) && a.sm) al = al && Aligned16(a.sm);
+  if (al) {
+    constexpr int IT = 2;
+    const uint64_t blocks = (a.n + (uint64_t)kBlock * IT - 1) / ((uint64_t)kBlock * IT);
+    hipLaunchKernelGGL((ShapeQueryKernel<S, XK, Q, IT>), dim3((unsigned)blocks), dim3(kBlock), 0, s, P, X, a);
+  } else {
+    const uint64_t blocks = (a.n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL((ShapeQueryKernel<S, XK, Q, 1>), dim3((unsigned)blocks), dim3(kBlock), 0, s, P, X, a);
+  }
+  return CheckHip(hipGetLastError(), "shape query launch");
+}
+
+template <class S, int Q>
+int LaunchXf(const typename S::P &P, const PlacedRec &pr, const QArgs &a, hipStream_t s) {
+  switch (pr.xfkind) {
+    case kXfIdentity: return LaunchQ<S, kXfIdentity, Q>(P, pr.xf, a, s);
+    case kXfTranslation: return LaunchQ<S, kXfTranslation, Q>(P, pr.xf, a, s);
+    case kXfRotation: return LaunchQ<S, kXfRotation, Q>(P, pr.xf, a, s);
+    default: return LaunchQ<S, kXfGeneral, Q>(P, pr.xf, a, s);
+  }
+}
+
+template <class S>
+int LaunchShape(const double *block, const PlacedRec &pr, int q, const QArgs &a, hipStream_t s) {
+  typename S::P P;
+  std::memcpy(&P, block, sizeof(P));
+  switch (q) {
+    case kQContains: return LaunchXf<S, kQContains>(P, pr, a, s);
+    case kQInside: return LaunchXf<S, kQInside>(P, pr, a, s);
+    case kQDistIn: return LaunchXf<S, kQDistIn>(P, pr, a, s);
+    case kQDistInMin: return LaunchXf<S, kQDistInMin>(P, pr, a, s);
+    case kQDistOut: return LaunchQ<S, kXfIdentity, kQDistOut>(P, pr.xf, a, s);
+    case kQPlacedDistOut: return LaunchXf<S, kQPlacedDistOut>(P, pr, a, s);
+    case kQSafeIn: return LaunchXf<S, kQSafeIn>(P, pr, a, s);
+    case kQSafeInMin: return LaunchXf<S, kQSafeInMin>(P, pr, a, s);
+    case kQSafeOut: return LaunchQ<S, kXfIdentity, kQSafeOut>(P, pr.xf, a, s);
+    case kQSafeOutMin: return LaunchQ<S, kXfIdentity, kQSafeOutMin>(P, pr.xf, a, s);
+  }
+  return SetError(VG_ERR_INVALID, "unknown query");
+}
+
+}  // namespace
+
+bool ShapeHasKernels(int type) { return type == VG_BOX || type == VG_TUBE; }
+
+int LaunchQuery(const vg_geometry *g, int32_t placed, int q, const QArgs &a, hipStream_t s) {
+  const PlacedRec &pr = g->placed[placed];
+  const double *block = g->params.data() + pr.param_off;
+  switch (pr.type) {
+    case VG_BOX: return LaunchShape<BoxS>(block, pr, q, a, s);
+    case VG_TUBE: return LaunchShape<TubeS>(block, pr, q, a, s);
+  }
+  return SetError(VG_ERR_UNSUPPORTED, "no device kernels for shape type " + std::to_string(pr.type));
+}
+
+}  // namespace vga
