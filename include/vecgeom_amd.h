@@ -64,7 +64,7 @@ int vg_shutdown(void);
  *   VG_TRD            dx1 dx2 dy1 dy2 dz               (UnplacedTrd.h:73)
  *   VG_TRAPEZOID      dz theta phi dy1 dx1 dx2 tanAlpha1 dy2 dx3 dx4 tanAlpha2
  *                                                      (UnplacedTrapezoid.h:93)
- *   VG_PARALLELEPIPED x y z alpha theta phi            (UnplacedParallelepiped.h:34)
+ *   VG_PARALLELEPIPED x y z alpha theta phi [deg]      (UnplacedParallelepiped.h:34)
  *   VG_PARABOLOID     rlo rhi dz                       (UnplacedParaboloid.h:85)
  *   VG_TORUS          rmin rmax rtor sphi dphi [rad]   (UnplacedTorus.h:61)
  *   VG_POLYCONE       sphi dphi nz rmin[nz] rmax[nz] z[nz] (UnplacedPolycone.h:82)

@@ -47,9 +47,10 @@ struct RefPlaced {
 // (volumes/PlacedVolume.h:260-263 asserts); this bound of its 8 corners is
 // the convention shared with the oracle and the product.
 void ParallelepipedExtent(const std::vector<double> &p, double *lo, double *hi) {
-  const double ta = std::tan(p[3]), tt = std::tan(p[4]);
-  const double ex = p[0] + std::fabs(p[1] * ta) + std::fabs(p[2] * tt * std::cos(p[5]));
-  const double ey = p[1] + std::fabs(p[2] * tt * std::sin(p[5]));
+  const double k = kDegToRad;  // angles are in degrees (source/UnplacedParallelepiped.cpp:48-70)
+  const double ta = std::tan(k * p[3]), tt = std::tan(k * p[4]);
+  const double ex = p[0] + std::fabs(p[1] * ta) + std::fabs(p[2] * tt * std::cos(k * p[5]));
+  const double ey = p[1] + std::fabs(p[2] * tt * std::sin(k * p[5]));
   const double ez = p[2];
   lo[0] = -ex; lo[1] = -ey; lo[2] = -ez;
   hi[0] = ex; hi[1] = ey; hi[2] = ez;
@@ -59,7 +60,16 @@ VUnplacedVolume *MakeUnplaced(int type, const double *p, int n) {
   switch (type) {
     case 1: if (n == 3) return new UnplacedBox(p[0], p[1], p[2]); break;
     case 2: if (n == 5) return new UnplacedTube(p[0], p[1], p[2], p[3], p[4]); break;
-    case 3: if (n == 1) return new UnplacedOrb(p[0]); break;
+    case 3:
+      if (n == 1) {
+        // UnplacedOrb's constructor reads frTolerance before initialising it
+        // (volumes/UnplacedOrb.h:29-36, source/UnplacedOrb.cpp:35-43);
+        // SetRadius recomputes the tolerances from initialised members.
+        auto *o = new UnplacedOrb(p[0]);
+        o->SetRadius(p[0]);
+        return o;
+      }
+      break;
     case 4: if (n == 6) return new UnplacedSphere(p[0], p[1], p[2], p[3], p[4], p[5]); break;
     case 5: if (n == 7) return new UnplacedCone(p[0], p[1], p[2], p[3], p[4], p[5], p[6]); break;
     case 6: if (n == 5) return new UnplacedTrd(p[0], p[1], p[2], p[3], p[4]); break;

@@ -1,3 +1,5 @@
-// All device shape kernels and the per-type dispatch helpers.
+// All device shape kernels.
 #pragma once
 #include "shapes_box_tube.hpp"
+#include "shapes_simple.hpp"
+#include "shapes_trap.hpp"

@@ -228,7 +228,14 @@ int LaunchShape(const double *block, const PlacedRec &pr, int q, const QArgs &a,
 
 }  // namespace
 
-bool ShapeHasKernels(int type) { return type == VG_BOX || type == VG_TUBE; }
+bool ShapeHasKernels(int type) {
+  switch (type) {
+    case VG_BOX: case VG_TUBE: case VG_ORB: case VG_TRD: case VG_PARALLELEPIPED: case VG_PARABOLOID:
+    case VG_TRAPEZOID:
+      return true;
+  }
+  return false;
+}
 
 int LaunchQuery(const vg_geometry *g, int32_t placed, int q, const QArgs &a, hipStream_t s) {
   const PlacedRec &pr = g->placed[placed];
@@ -236,6 +243,11 @@ int LaunchQuery(const vg_geometry *g, int32_t placed, int q, const QArgs &a, hip
   switch (pr.type) {
     case VG_BOX: return LaunchShape<BoxS>(block, pr, q, a, s);
     case VG_TUBE: return LaunchShape<TubeS>(block, pr, q, a, s);
+    case VG_ORB: return LaunchShape<OrbS>(block, pr, q, a, s);
+    case VG_TRD: return LaunchShape<TrdS>(block, pr, q, a, s);
+    case VG_PARALLELEPIPED: return LaunchShape<ParaS>(block, pr, q, a, s);
+    case VG_PARABOLOID: return LaunchShape<ParabS>(block, pr, q, a, s);
+    case VG_TRAPEZOID: return LaunchShape<TrapS>(block, pr, q, a, s);
   }
   return SetError(VG_ERR_UNSUPPORTED, "no device kernels for shape type " + std::to_string(pr.type));
 }

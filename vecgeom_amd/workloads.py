@@ -36,8 +36,7 @@ def zoo_shapes():
         ("Cone", G.UnplacedCone(5.0, 20.0, 10.0, 40.0, 30.0, 0.0, TWO_PI)),
         ("Trd", G.UnplacedTrd(15.0, 20.0, 25.0, 30.0, 35.0)),
         ("Trapezoid", G.UnplacedTrapezoid(40.0, 0.1, 0.3, 20.0, 15.0, 25.0, 0.1, 30.0, 20.0, 35.0, 0.1)),
-        ("Parallelepiped", G.UnplacedParallelepiped(20.0, 30.0, 40.0, math.radians(30.0), math.radians(15.0),
-                                                    math.radians(30.0))),
+        ("Parallelepiped", G.UnplacedParallelepiped(20.0, 30.0, 40.0, 30.0, 15.0, 30.0)),  # degrees
         ("Paraboloid", G.UnplacedParaboloid(10.0, 30.0, 40.0)),
         ("Torus", G.UnplacedTorus(5.0, 10.0, 50.0, 0.0, TWO_PI)),
         ("Polycone", G.UnplacedPolycone(
