@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--points", type=int, default=1 << 22)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-FP64-form denominator")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run a device-to-device copy of MIB MiB (for HBM counter scaling)")
     return ap.parse_args()
 
 
@@ -112,6 +115,32 @@ class Workload:
         return out
 
 
+def time_steps(launches, steps, stream, watch=None):
+    """Runs `steps` steps and returns (elapsed seconds, {launch index: [ms]}).
+
+    HIP events are recorded on the launch stream around the launches whose
+    indices are in `watch` (None = every launch); keeping them off the other
+    launches keeps the event records out of the throughput measurement."""
+    import torch
+
+    idx = range(len(launches)) if watch is None else list(watch)
+    ev = {k: [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+          for k in idx}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        for k, (key, fn) in enumerate(launches):
+            e = ev.get(k)
+            if e is not None:
+                e[s][0].record(stream)
+            fn()
+            if e is not None:
+                e[s][1].record(stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    return elapsed, {k: [a.elapsed_time(b) for a, b in v] for k, v in ev.items()}
+
+
 def cpu_baseline(wl, target_s):
     """Oracle (C++ restatement of the reference Scalar kernels) on the host:
     the same queries on a bounded, evenly strided sample of the same inputs."""
@@ -123,11 +152,12 @@ def cpu_baseline(wl, target_s):
     # Calibrate: one shape, all queries, small sample.
     frac = 1.0 / 64
     total_q = 0
+    reps = 1
     t0 = time.perf_counter()
     while True:
         total_q = 0
         t0 = time.perf_counter()
-        for it, (name, u), xf in zip(wl.items, wl.shapes, wl.xfs):
+        for it, (name, u), xf in [z for _ in range(reps) for z in zip(wl.items, wl.shapes, wl.xfs)]:
             h = o.placed(u.type, u.params, xf.array())
             hp = it["host"]
             step = int(round(1 / frac))
@@ -143,15 +173,18 @@ def cpu_baseline(wl, target_s):
             h.query(po.Q_SAFE_OUT, LP[:, iin], nthreads=nthreads)
             total_q += 2 * len(range(0, wl.n, step)) + 2 * len(iout) + 2 * len(iin)
         dt = time.perf_counter() - t0
-        if dt >= 0.5 * target_s or frac >= 1.0:
+        if dt >= 0.5 * target_s:
             break
-        frac = min(1.0, frac * max(2.0, target_s / max(dt, 1e-3)))
+        if frac >= 1.0:
+            reps = max(reps + 1, int(reps * target_s / max(dt, 1e-3)))
+        else:
+            frac = min(1.0, frac * max(2.0, target_s / max(dt, 1e-3)))
     return {
         "value": total_q / dt / 1e9,
         "unit": "Gqueries/s",
         "cores": nthreads,
         "kind": "port",
-        "sample": f"every {int(round(1 / frac))}th query of the same workload ({total_q} queries, {dt:.1f} s), "
+        "sample": f"every {int(round(1 / frac))}th query of the same workload x{reps} ({total_q} queries, {dt:.1f} s), "
                   f"oracle C++ restatement of the reference Scalar backend, std::thread over contiguous ranges",
     }
 
@@ -176,42 +209,62 @@ def main():
         for _, fn in launches:
             fn()
 
+    if args.copy_calibration > 0:
+        # A plain device-to-device copy of known size, so that HBM counters
+        # (FETCH_SIZE / WRITE_SIZE) can be scaled against it.
+        nbytes = args.copy_calibration << 20
+        src = torch.empty(nbytes // 8, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
 
-    # Timed region: K steps, with HIP events around each launch on the launch
-    # stream (torch's current stream, which the C-ABI calls use).
-    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in launches]
-          for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        for (key, fn), (a, b) in zip(launches, ev[s]):
-            a.record(stream)
-            fn()
-            b.record(stream)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    # Census (untimed): HIP events around every launch give the per-kernel
+    # breakdown and pick the dominant kernel.
+    _, cen = time_steps(launches, 3, stream)
+    keys = [key for key, _ in launches]
+    count = {(it["name"], q): c for it in wl.items for q, c in it["count"].items()}
+    rows = []
+    for k, key in enumerate(keys):
+        ms = float(np.mean(cen[k]))
+        cnt = count[key]
+        rows.append({"shape": key[0], "query": key[1], "n": cnt, "avg_us": 1e3 * ms,
+                     "gqps": cnt / (ms * 1e-3) / 1e9 if ms > 0 else None,
+                     "gbs": cnt * BYTES[key[1]] / (ms * 1e-3) / 1e9 if ms > 0 else None})
+    kdom = max(range(len(rows)), key=lambda k: rows[k]["avg_us"])
+
+    # Timed region: K steps; HIP events on the launch stream (torch's current
+    # stream, which the C-ABI calls use) around the dominant kernel only.
+    elapsed, ev = time_steps(launches, args.steps, stream, watch=[kdom])
     ms_per_step = 1e3 * elapsed / args.steps
     value = wl.queries_per_step * args.steps / elapsed / 1e9
+    dom = dict(rows[kdom])
+    dom_ms = float(np.mean(ev[kdom]))
+    dom_bytes = dom["n"] * BYTES[dom["query"]]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": f"ShapeQueryKernel<{dom['shape']},{dom['query']}>", "avg_launch_us": round(1e3 * dom_ms, 3),
+            "bytes_per_query": BYTES[dom["query"]], "queries_per_launch": dom["n"], "bytes_per_launch": dom_bytes}
 
-    per = {}
-    for s in range(args.steps):
-        for (key, fn), (a, b) in zip(launches, ev[s]):
-            per.setdefault(key, []).append(a.elapsed_time(b))
-    rows = []
-    for it in wl.items:
-        for q, cnt in it["count"].items():
-            ms = float(np.mean(per[(it["name"], q)]))
-            rows.append({"shape": it["name"], "query": q, "n": cnt, "avg_us": 1e3 * ms,
-                         "gqps": cnt / (ms * 1e-3) / 1e9 if ms > 0 else None,
-                         "gbs": cnt * BYTES[q] / (ms * 1e-3) / 1e9 if ms > 0 else None})
-    dom = max(rows, key=lambda r: r["avg_us"] * 1.0)
-    roof = {"bound": "hbm", "achieved": round(dom["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": f"ShapeQueryKernel<{dom['shape']},{dom['query']}>",
-            "bytes_per_query": BYTES[dom["query"]], "queries_per_launch": dom["n"]}
+    plain = None
+    if not args.no_plain:
+        # The plain FP64 GPU form: same queries, one thread per query with
+        # parameters chased from global memory (vg_set_kernel_variant(1)).
+        from vecgeom_amd import capi
+        capi.call("vg_set_kernel_variant", 1)
+        for _ in range(max(1, args.warmup)):
+            step()
+        pel, _ = time_steps(launches, args.steps, stream, watch=[])
+        capi.call("vg_set_kernel_variant", 0)
+        pval = wl.queries_per_step * args.steps / pel / 1e9
+        plain = {"value": round(pval, 3), "unit": "Gqueries/s", "ms_per_step": round(1e3 * pel / args.steps, 4),
+                 "speedup": round(value / pval, 3)}
 
     cpu = None
     if not args.no_cpu:
@@ -226,6 +279,7 @@ def main():
                    "points_per_shape": args.points, "shapes": [nm for nm, _ in wl.shapes],
                    "shapes_without_kernels": wl.missing, "queries_per_step": wl.queries_per_step},
         "roofline": roof,
+        "plain_fp64_gpu": plain,
         "cpu_baseline": cpu,
         "per_kernel": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()} for r in rows],
     }

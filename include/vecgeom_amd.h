@@ -89,6 +89,14 @@ enum vg_shape_type {
 /* Returns 1 if the library has device kernels for this shape type. */
 int vg_shape_supported(int32_t shape_type);
 
+/* Selects the kernels used by the basket queries of this process:
+ * 0 = the tuned kernels (default); 1 = the plain FP64 form, one thread per
+ * query with the placement and shape parameters read from global memory per
+ * thread, as the reference's CUDA benchmark kernels do with a device virtual
+ * call (source/benchmarking/Benchmarker.cu:13-79).  The plain form exists as
+ * the benchmark denominator; results are identical. */
+int vg_set_kernel_variant(int variant);
+
 /* ------------------------------------------------------ geometry upload */
 /*
  * Flattened volume hierarchy.  Host pointers; the library copies what it

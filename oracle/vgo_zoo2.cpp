@@ -4,7 +4,378 @@
 
 namespace vgo {
 Shape *MakeCone(const double *, int) { return nullptr; }
-Shape *MakeSphere(const double *, int) { return nullptr; }
+// ---------------------------------------------------------------- Sphere
+// volumes/kernel/SphereImplementation.h:257-1520, volumes/UnplacedSphere.h:
+// 63-215 (angle checks, cached trigonometry), source/UnplacedSphere.cpp:25-95.
+// ATan2 follows backend/scalar/Backend.h:130-137.
+inline double ATan2(double y, double x) {
+  if (x != 0) return std::atan2(y, x);
+  if (y > 0) return kPi / 2;
+  if (y < 0) return -kPi / 2;
+  return 0;
+}
+
+struct Sphere : Shape {
+  double fRmin, fRmax, fSPhi = 0, fDPhi = 0, fSTheta = 0, fDTheta = 0;
+  double fRminTolerance, mkTolerance, kAngTol = 1e-9;
+  double sinCPhi, cosCPhi, cosHDPhiOT, cosHDPhiIT, sinSPhi, cosSPhi, sinEPhi, cosEPhi, hDPhi, cPhi, ePhi;
+  double sinSTheta, cosSTheta, sinETheta, cosETheta, tanSTheta, tanSTheta2, tanETheta, tanETheta2, eTheta;
+  bool fullPhi = true, fullTheta = true;
+  Sphere(const double *p) {
+    fRmin = p[0];
+    fRmax = p[1];
+    fRminTolerance = (fRmin) ? Max(1e-9, 2.e-11 * fRmin) : 0;
+    mkTolerance = Max(1e-9, 2.e-11 * fRmax);
+    // CheckPhiAngles
+    fullPhi = true;
+    if (p[3] >= 2 * kPi - kAngTol * 0.5) {
+      fDPhi = 2 * kPi;
+      fSPhi = 0;
+    } else {
+      fullPhi = false;
+      if (p[3] > 0) fDPhi = p[3];
+    }
+    if (!fullPhi) {
+      double sPhi = p[2];
+      if (sPhi < 0)
+        fSPhi = 2 * kPi - std::fmod(std::fabs(sPhi), 2 * kPi);
+      else
+        fSPhi = std::fmod(sPhi, 2 * kPi);
+      if (fSPhi + fDPhi > 2 * kPi) fSPhi -= 2 * kPi;
+    }
+    hDPhi = 0.5 * fDPhi;
+    cPhi = fSPhi + hDPhi;
+    ePhi = fSPhi + fDPhi;
+    sinCPhi = std::sin(cPhi);
+    cosCPhi = std::cos(cPhi);
+    cosHDPhiIT = std::cos(hDPhi - 0.5 * kAngTol);
+    cosHDPhiOT = std::cos(hDPhi + 0.5 * kAngTol);
+    sinSPhi = std::sin(fSPhi);
+    cosSPhi = std::cos(fSPhi);
+    sinEPhi = std::sin(ePhi);
+    cosEPhi = std::cos(ePhi);
+    // CheckThetaAngles
+    double sTheta = p[4], dTheta = p[5];
+    if (!((sTheta < 0) || (sTheta > kPi))) fSTheta = sTheta;
+    if (dTheta + sTheta >= kPi)
+      fDTheta = kPi - sTheta;
+    else if (dTheta > 0)
+      fDTheta = dTheta;
+    fullTheta = !(fDTheta - fSTheta < kPi);
+    eTheta = fSTheta + fDTheta;
+    sinSTheta = std::sin(fSTheta);
+    cosSTheta = std::cos(fSTheta);
+    sinETheta = std::sin(eTheta);
+    cosETheta = std::cos(eTheta);
+    tanSTheta = std::tan(fSTheta);
+    tanSTheta2 = tanSTheta * tanSTheta;
+    tanETheta = std::tan(eTheta);
+    tanETheta2 = tanETheta * tanETheta;
+  }
+  // GenericKernelForContainsAndInside :801-893 (the body does not depend on
+  // ForInside)
+  void Kernel(V3 p, bool &cin, bool &cout) const {
+    double half = 0.5 * kAngTol;
+    double rad2 = p.Mag2();
+    double tolRMin = fRmin + (0.5 * fRminTolerance);
+    double tolRMax = fRmax - (0.5 * fRminTolerance);
+    cin = (rad2 <= tolRMax * tolRMax) && (rad2 >= tolRMin * tolRMin);
+    tolRMin = fRmin - (0.5 * fRminTolerance);
+    tolRMax = fRmax + (0.5 * fRminTolerance);
+    cout = (rad2 <= tolRMin * tolRMin) || (rad2 >= tolRMax * tolRMax);
+    if (cout) return;
+    double pPhi = ATan2(p.y, p.x);
+    double eph = fSPhi + fDPhi;
+    if (pPhi < (fSPhi - half)) pPhi = pPhi + (2. * kPi);
+    if (pPhi > (eph + half)) pPhi = pPhi - (2. * kPi);
+    double tolAngMin = fSPhi + half, tolAngMax = eph - half;
+    if (!fullPhi) {
+      cin &= (pPhi <= tolAngMax) && (pPhi >= tolAngMin);
+      tolAngMin = fSPhi - half;
+      tolAngMax = eph + half;
+      cout |= (pPhi < tolAngMin) || (pPhi > tolAngMax);
+      if (cout) return;
+    }
+    double pTheta = ATan2(Sqrt(p.x * p.x + p.y * p.y), p.z);
+    double eth = fSTheta + fDTheta;
+    tolAngMin = fSTheta + half;
+    tolAngMax = eth - half;
+    if (!fullTheta) {
+      cin &= (pTheta <= tolAngMax) && (pTheta >= tolAngMin);
+      tolAngMin = fSTheta - half;
+      tolAngMax = eth + half;
+      cout |= (pTheta < tolAngMin) || (pTheta > tolAngMax);
+    }
+  }
+  bool Contains(V3 p) const override {
+    bool cin, cout;
+    Kernel(p, cin, cout);
+    return !cout;
+  }
+  int Inside(V3 p) const override {
+    bool cin, cout;
+    Kernel(p, cin, cout);
+    int r = kSurface;
+    if (cout) r = kOutside;
+    if (cin) r = kInside;
+    return r;
+  }
+  // The reference's DistanceToIn is an empty body (:85-94): the helper's
+  // initial kInfinity is returned.
+  double DistanceToIn(V3, V3, double) const override { return kInfinity; }
+  // SafetyToInKernel :945-1040
+  double SafetyToIn(V3 p) const override {
+    double safety = kInfinity;
+    double rad = Sqrt(p.Mag2());
+    double rho2 = p.x * p.x + p.y * p.y;
+    double rho = std::sqrt(rho2);
+    double safeRMin = fRmin - rad, safeRMax = rad - fRmax;
+    if ((fRmin > 0) && (safeRMin > safeRMax)) safety = safeRMin;
+    if ((fRmin > 0) && !(safeRMin > safeRMax)) safety = safeRMax;
+    if (!(fRmin > 0)) safety = safeRMax;
+    double safePhi = 0;
+    double cosPsi = (p.x * cosCPhi + p.y * sinCPhi) / rho;
+    if (!fullPhi && (rho > 0)) {
+      double test1 = (p.x * sinSPhi - p.y * cosSPhi);
+      if (test1 < 0) test1 = -1. * test1;
+      double test2 = (p.x * sinEPhi - p.y * cosEPhi);
+      if (test2 < 0) test2 = -1. * test2;
+      bool c = cosPsi < std::cos(hDPhi);
+      bool side = (p.y * cosCPhi - p.x * sinCPhi) <= 0;
+      if (c && side) safePhi = test1;
+      if (c && !side) safePhi = test2;
+      if (c && (safePhi > safety)) safety = safePhi;
+    }
+    double rds = p.Mag();
+    double pTheta = (kPi / 2) - std::asin(p.z / rds);
+    if (pTheta < 0) pTheta = pTheta + kPi;
+    double dTheta1 = fSTheta - pTheta, dTheta2 = pTheta - eTheta;
+    double sinDTheta1 = std::sin(dTheta1), sinDTheta2 = std::sin(dTheta2);
+    double safeTheta = 0;
+    if (!fullTheta && (rds != 0)) {
+      bool a = dTheta1 > dTheta2;
+      if (a && (dTheta1 >= 0)) safeTheta = rds * sinDTheta1;
+      if (a && (dTheta1 >= 0) && (safety <= safeTheta)) safety = safeTheta;
+      if (!a && (dTheta2 >= 0)) safeTheta = rds * sinDTheta2;
+      if (!a && (dTheta2 >= 0) && (safety <= safeTheta)) safety = safeTheta;
+    }
+    if (safety < 0) safety = 0;
+    return safety;
+  }
+  // SafetyToOutKernel :1053-1120
+  double SafetyToOut(V3 p) const override {
+    double safety = kInfinity;
+    double rad = Sqrt(p.Mag2());
+    double rho = Sqrt(p.x * p.x + p.y * p.y);
+    double safeRMin = rad - fRmin, safeRMax = fRmax - rad;
+    if ((fRmin > 0) && (safeRMin < safeRMax)) safety = safeRMin;
+    if ((fRmin > 0) && !(safeRMin < safeRMax)) safety = safeRMax;
+    if (!(fRmin > 0)) safety = safeRMax;
+    double safePhi = 0;
+    if (!fullPhi && (rho > 0)) {
+      double test1 = (p.y * cosCPhi - p.x * sinCPhi);
+      if (test1 <= 0) safePhi = -1. * (p.x * sinSPhi - p.y * cosSPhi);
+      if (!(test1 <= 0)) safePhi = (p.x * sinEPhi - p.y * cosEPhi);
+      if (safePhi < safety) safety = safePhi;
+    }
+    double rds = p.Mag();
+    double pTheta = (kPi / 2) - std::asin(p.z / rds);
+    if (pTheta < 0) pTheta = pTheta + kPi;
+    double eth = fSTheta + fDTheta;
+    double dTheta1 = pTheta - fSTheta, dTheta2 = eth - pTheta;
+    double sinDTheta1 = std::sin(dTheta1), sinDTheta2 = std::sin(dTheta2);
+    double safeTheta = 0;
+    if (!fullTheta && (rds > 0)) {
+      if (dTheta1 < dTheta2) safeTheta = rds * sinDTheta1;
+      if (!(dTheta1 < dTheta2)) safeTheta = rds * sinDTheta2;
+      if (safety > safeTheta) safety = safeTheta;
+    }
+    if (safety < 0) safety = 0;
+    return safety;
+  }
+  // DistanceToSphere :384-446
+  double ToSphere(V3 p, V3 d, double radius, bool check, bool firstcross) const {
+    double rad2 = p.Mag2();
+    double pDotV = Dot(p, d);
+    double c = rad2 - radius * radius;
+    double d2 = pDotV * pDotV - c;
+    double dd = Sqrt(d2);
+    if (d2 <= 0) return kInfinity;
+    double distance;
+    bool in = (c <= 0);
+    if (in) distance = -1. * pDotV + dd;
+    else if (firstcross) distance = -1. * pDotV - dd;
+    else distance = -1. * pDotV + dd;
+    if (distance < 0) distance = kInfinity;
+    V3 pt = p + distance * d;
+    bool cin, cout;
+    Kernel(pt, cin, cout);
+    if (check && cout) distance = kInfinity;
+    return distance;
+  }
+  // DistanceToCone :314-382
+  void ToCone(V3 p, V3 d, double dzv, double r1v, double r2v, double &bv, double &deltav) const {
+    deltav = -1.;
+    if (dzv < 0) return;
+    double ro0v = (0.5 * (r1v + r2v));
+    double tzv = (0.5 * (r2v - r1v) / dzv);
+    double rsqv = p.x * p.x + p.y * p.y;
+    double rcv = ro0v + p.z * tzv;
+    double av = d.x * d.x + d.y * d.y - tzv * tzv * d.z * d.z;
+    bv = p.x * d.x + p.y * d.y - tzv * rcv * d.z;
+    double cv = rsqv - rcv * rcv;
+    double t1 = av, t2 = bv;
+    if (t1 < 0) t1 = -1. * t1;
+    if (t2 < 0) t2 = -1. * t2;
+    const double toler = 1.e-10;
+    if ((t1 < toler) && (t2 < toler)) return;
+    if (t1 < toler) {
+      bv = 0.5 * (cv / bv);
+      deltav = 0;
+      return;
+    }
+    av = (1. / av);
+    bv *= av;
+    cv *= av;
+    deltav = bv * bv - cv;
+    if (deltav > 0)
+      deltav = Sqrt(deltav);
+    else
+      deltav = -1.;
+  }
+  // DistanceToPhiMin :257-312 (in = true)
+  double ToPhiMin(V3 p, V3 d, double s1, double c1, double s2, double c2, double sm, double cm) const {
+    double sfi1 = kInfinity, sfi2 = kInfinity, s = 0;
+    double un = d.x * s1 - d.y * c1;
+    if (un > 0) s = (-1. * p.x * s1 + p.y * c1);
+    if ((un > 0) && (s >= 0)) s = (s / un);
+    if ((un > 0) && (s >= 0) && (((p.x + s * d.x) * sm - (p.y + s * d.y) * cm) >= 0)) sfi1 = s;
+    un = -1. * d.x * s2 + d.y * c2;
+    if (un > 0) s = (p.x * s2 - p.y * c2);
+    if ((un > 0) && (s >= 0)) s = (s / un);
+    if ((un > 0) && (s >= 0) && ((-1. * (p.x + s * d.x) * sm + (p.y + s * d.y) * cm) >= 0)) sfi2 = s;
+    return Min(sfi1, sfi2);
+  }
+  // One theta cone block of DistanceToOutKernel :1336-1478; returns false
+  // when the kernel returns 0 early (on the cone surface, moving out).
+  bool ThetaBlock(V3 p, V3 d, double rxy2, bool cond, double si, double ci, bool first, double fPhi1,
+                  double fPhi2, double &b, double &delta, double &snxt, double &sn) const {
+    const double toler = 1.E-10, toler2 = 1.E+10;
+    double r1 = 0, r2 = 0, z1 = 0, z2 = 0;
+    if (cond && (ci > 0)) { r1 = fRmin * si; z1 = fRmin * ci; r2 = fRmax * si; z2 = fRmax * ci; }
+    if (cond && !(ci > 0)) { r1 = fRmax * si; z1 = fRmax * ci; r2 = fRmin * si; z2 = fRmin * ci; }
+    double dz = 0.5 * (z2 - z1);
+    V3 ptnew = p;
+    ptnew.z = p.z - 0.5 * (z1 + z2);
+    double zinv = 1. / dz;
+    double rin = 0.5 * (r1 + r2 + (r2 - r1) * ptnew.z * zinv);
+    double sigz = 1.;
+    if (p.z < 0) sigz = -1. * sigz;
+    bool bigequ = first ? ((sigz * ci > 0) && (sigz * rxy2 < sigz * rin * (rin + sigz * toler)))
+                        : ((sigz * ci > 0) && (sigz * rxy2 > sigz * rin * (rin - sigz * toler)));
+    double ddotn = 0;
+    double ddotnTemp = ptnew.x * d.x + ptnew.y * d.y + 0.5 * (r1 - r2) * d.z * zinv * Sqrt(rxy2);
+    if (cond && bigequ) ddotn = ddotnTemp;
+    if (first) {
+      if (cond && bigequ && (sigz * ddotn <= 0)) return false;
+    } else {
+      if (cond && bigequ && (sigz * ddotn >= 0)) return false;
+    }
+    double bv = 0, deltav = 0;
+    ToCone(ptnew, d, dz, r1, r2, bv, deltav);
+    bool ok = cond && !bigequ;
+    if (ok) { delta = deltav; b = bv; }
+    double xnew, ynew, znew, phi0, ddp, temp;
+    if (ok && (delta > 0)) {
+      snxt = -1. * b - delta;
+      znew = ptnew.z + snxt * d.z;
+      temp = znew < 0 ? -1. * znew : znew;
+      if ((snxt > 0) && (temp < dz)) {
+        if (fullPhi) {
+          sn = snxt;
+        } else {
+          xnew = ptnew.x + snxt * d.x;
+          ynew = ptnew.y + snxt * d.y;
+          phi0 = ATan2(ynew, xnew);
+          ddp = phi0 - fPhi1;
+          if (ddp < 0) ddp = ddp + 2 * kPi;
+          if (ddp <= (fPhi2 - fPhi1)) sn = snxt;
+        }
+      }
+      // The second root: block one tests sn > toler2 for both the root and
+      // its use; block two computes the root when sn > toler and uses it
+      // when sn > toler2 (:1404-1406 and :1466-1468).
+      bool compute = first ? (sn > toler2) : (sn > toler);
+      if (compute) snxt = -1. * b + delta;
+      if (sn > toler2) {
+        znew = ptnew.z + snxt * d.z;
+        temp = znew < 0 ? -1. * znew : znew;
+        if ((snxt > 0) && (temp < dz)) {
+          if (fullPhi) {
+            sn = snxt;
+          } else {
+            xnew = ptnew.x + snxt * d.x;
+            ynew = ptnew.y + snxt * d.y;
+            phi0 = ATan2(ynew, xnew);
+            ddp = phi0 - fPhi1;
+            if (ddp < 0) ddp = ddp + 2 * kPi;
+            if (ddp <= (fPhi2 - fPhi1)) sn = snxt;
+          }
+        }
+      }
+    }
+    return true;
+  }
+  // DistanceToOutKernel :1179-1520
+  double DistanceToOut(V3 p, V3 d, double) const override {
+    const double toler = 1.E-10;
+    double rad = Sqrt(p.Mag2());
+    double pDotV = Dot(p, d);
+    double rxy2 = p.x * p.x + p.y * p.y;
+    double fPhi1 = fSPhi, fPhi2 = ePhi;
+    double sn1 = kInfinity, rn1 = kInfinity, sn2 = kInfinity, rn2 = kInfinity;
+    if ((fRmin > 0) && (rad <= (fRmin + toler)) && (pDotV < 0)) return kInfinity;
+    double dist = ToSphere(p, d, fRmin, true, true);
+    if ((fRmin > 0) && !(rad <= (fRmin + toler)) && (pDotV < 0)) rn1 = dist;
+    if ((rad >= (fRmax - toler)) && (pDotV >= 0)) return kInfinity;
+    rn2 = ToSphere(p, d, fRmax, false, false);
+    double sr = Min(rn1, rn2);
+    double fTheta1 = fSTheta, fTheta2 = eTheta;
+    double theta2Plane = std::abs(fTheta2 - kPi / 2);
+    double b = 0, delta = 0, snxt = kInfinity;
+    if (!fullTheta) {
+      double theta1Plane = std::abs(fTheta1 - kPi / 2);
+      if (theta1Plane < 1.e-10) {
+        if ((p.z * d.z) < 0) sn1 = -1. * p.z / d.z;
+      } else {
+        if (!ThetaBlock(p, d, rxy2, fTheta1 > 0, sinSTheta, cosSTheta, true, fPhi1, fPhi2, b, delta, snxt, sn1))
+          return 0.;
+      }
+      snxt = kInfinity;
+      if (theta2Plane < 1.e-10) {
+        if ((p.z * d.z) < 0) sn1 = -1. * p.z / d.z;
+      } else {
+        if (!ThetaBlock(p, d, rxy2, fTheta2 < kPi, sinETheta, cosETheta, false, fPhi1, fPhi2, b, delta, snxt, sn2))
+          return 0.;
+      }
+    }
+    double st = Min(sn1, sn2);
+    double sp = kInfinity;
+    if (!fullPhi) {
+      double phim = 0.5 * (fPhi1 + fPhi2);
+      sp = ToPhiMin(p, d, sinSPhi, cosSPhi, sinEPhi, cosEPhi, std::sin(phim), std::cos(phim));
+    }
+    double srt = Min(sr, st);
+    return Min(srt, sp);
+  }
+  // UnplacedSphere::Extent, source/UnplacedSphere.cpp (full cube of rmax)
+  void Extent(V3 &lo, V3 &hi) const override {
+    lo = {-fRmax, -fRmax, -fRmax};
+    hi = {fRmax, fRmax, fRmax};
+  }
+};
+
+Shape *MakeSphere(const double *p, int n) { return n == 6 ? new Sphere(p) : nullptr; }
 // ---------------------------------------------------------------- Trapezoid
 // volumes/kernel/TrapezoidImplementation.h:140-470 with the PlaneShell
 // variant (VECGEOM_PLANESHELL_DISABLE is not set: base/PlaneShell.h:95-300);

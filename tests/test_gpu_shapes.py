@@ -204,3 +204,30 @@ def test_gpu_full_size_properties(gpu, oracle):
         assert same(c_np[sel], o.query(po.Q_CONTAINS, inp["p"][:, sel])).all()
         assert same(i_np[sel], o.query(po.Q_INSIDE, inp["p"][:, sel])).all()
         assert same(d_np[sel], o.query(po.Q_DIST_IN, inp["p"][:, sel], inp["d"][:, sel])).all()
+
+
+def test_gpu_plain_variant_matches_tuned(gpu):
+    """The plain FP64 form (benchmark denominator) gives identical results."""
+    shapes = shapes_under_test()
+    rng = np.random.Generator(np.random.Philox(8))
+    xfs = [W.random_placement(rng) for _ in shapes]
+    geom, ids, _ = G.single_shape_geometry([u for _, u in shapes], xfs)
+    n = 30001
+    try:
+        for (name, u), xf, pid in zip(shapes, xfs, ids):
+            inp = W.shape_inputs(geom.placed_extent(pid), xf, n, seed=55 + pid)
+            g = {"p": inp["p"], "d": inp["d"], "lp": inp["lp"], "ld": inp["ld"], "sm": np.full(n, 1e30)}
+            capi.call("vg_set_kernel_variant", 0)
+            tuned = run_all(geom, pid, g, gpu)
+            capi.call("vg_set_kernel_variant", 1)
+            p, d = dev_soa(g["p"], gpu), dev_soa(g["d"], gpu)
+            lp, ld = dev_soa(g["lp"], gpu), dev_soa(g["ld"], gpu)
+            plain = {
+                "contains": host(Q.contains(geom, pid, p)), "inside": host(Q.inside(geom, pid, p)),
+                "dist_in": host(Q.distance_to_in(geom, pid, p, d)), "dist_out": host(Q.distance_to_out(geom, pid, lp, ld)),
+                "safe_in": host(Q.safety_to_in(geom, pid, p)), "safe_out": host(Q.safety_to_out(geom, pid, lp)),
+            }
+            for qn, v in plain.items():
+                assert same(v, tuned[qn]).all(), f"{name} {qn}: plain form differs from tuned kernels"
+    finally:
+        capi.call("vg_set_kernel_variant", 0)

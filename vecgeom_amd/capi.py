@@ -40,6 +40,7 @@ _SIGS = {
     "vg_init": [ctypes.c_int],
     "vg_shutdown": [],
     "vg_shape_supported": [c_i32],
+    "vg_set_kernel_variant": [ctypes.c_int],
     "vg_geometry_upload": [ctypes.POINTER(FlatGeometry), ctypes.POINTER(c_vp)],
     "vg_geometry_free": [c_vp],
     "vg_geometry_max_depth": [c_vp, ctypes.POINTER(c_i32)],

@@ -3,3 +3,4 @@
 #include "shapes_box_tube.hpp"
 #include "shapes_simple.hpp"
 #include "shapes_trap.hpp"
+#include "shapes_sphere.hpp"

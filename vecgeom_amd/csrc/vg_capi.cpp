@@ -147,6 +147,8 @@ int vg_shutdown(void) {
 
 int vg_shape_supported(int32_t shape_type) { return ShapeHasKernels(shape_type) ? 1 : 0; }
 
+int vg_set_kernel_variant(int variant) { return SetKernelVariant(variant); }
+
 int vg_geometry_upload(const vg_flat_geometry *f, vg_geometry **out) {
   if (!f || !out) return SetError(VG_ERR_INVALID, "null argument");
   *out = nullptr;

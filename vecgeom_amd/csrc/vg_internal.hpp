@@ -97,6 +97,7 @@ struct QArgs {
 // Launches one basket query on placed volume `placed` (vg_shape_kernels.hip).
 int LaunchQuery(const vg_geometry *g, int32_t placed, int q, const QArgs &a, hipStream_t s);
 bool ShapeHasKernels(int type);
+int SetKernelVariant(int v);
 
 }  // namespace vga
 

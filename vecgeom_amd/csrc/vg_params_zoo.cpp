@@ -131,6 +131,56 @@ std::vector<double> BuildZooParams(int type, const double *c, int n, std::string
       q.dy = q.dx;
       return Pack(q);
     }
+    case VG_SPHERE: {
+      if (n != 6 || !(c[0] >= 0 && c[1] > c[0] && c[3] > 0 && c[5] > 0))
+        return Fail(err, "sphere needs 0 <= rmin < rmax, dphi > 0, dtheta > 0");
+      // source/UnplacedSphere.cpp:25-95, volumes/UnplacedSphere.h:63-215
+      SphereP q;
+      std::memset(&q, 0, sizeof(q));
+      const double kAngTol = 1e-9;
+      q.fRmin = c[0];
+      q.fRmax = c[1];
+      q.fRminTolerance = (q.fRmin) ? Max(1e-9, 2.e-11 * q.fRmin) : 0;
+      bool fullPhi = true;
+      if (c[3] >= 2 * kPi - kAngTol * 0.5) {
+        q.fDPhi = 2 * kPi;
+        q.fSPhi = 0;
+      } else {
+        fullPhi = false;
+        q.fDPhi = c[3];
+        const double sPhi = c[2];
+        q.fSPhi = sPhi < 0 ? 2 * kPi - std::fmod(std::fabs(sPhi), 2 * kPi) : std::fmod(sPhi, 2 * kPi);
+        if (q.fSPhi + q.fDPhi > 2 * kPi) q.fSPhi -= 2 * kPi;
+      }
+      q.hDPhi = 0.5 * q.fDPhi;
+      q.cPhi = q.fSPhi + q.hDPhi;
+      q.ePhi = q.fSPhi + q.fDPhi;
+      q.sinCPhi = std::sin(q.cPhi);
+      q.cosCPhi = std::cos(q.cPhi);
+      q.sinSPhi = std::sin(q.fSPhi);
+      q.cosSPhi = std::cos(q.fSPhi);
+      q.sinEPhi = std::sin(q.ePhi);
+      q.cosEPhi = std::cos(q.ePhi);
+      const double sTheta = c[4], dTheta = c[5];
+      if (!((sTheta < 0) || (sTheta > kPi))) q.fSTheta = sTheta;
+      if (dTheta + sTheta >= kPi)
+        q.fDTheta = kPi - sTheta;
+      else
+        q.fDTheta = dTheta;
+      const bool fullTheta = !(q.fDTheta - q.fSTheta < kPi);
+      q.eTheta = q.fSTheta + q.fDTheta;
+      q.sinSTheta = std::sin(q.fSTheta);
+      q.cosSTheta = std::cos(q.fSTheta);
+      q.sinETheta = std::sin(q.eTheta);
+      q.cosETheta = std::cos(q.eTheta);
+      q.cosHDPhi = std::cos(q.hDPhi);
+      const double phim = 0.5 * (q.fSPhi + q.ePhi);
+      q.sinPhim = std::sin(phim);
+      q.cosPhim = std::cos(phim);
+      q.fullPhi = fullPhi ? 1. : 0.;
+      q.fullTheta = fullTheta ? 1. : 0.;
+      return Pack(q);
+    }
     case VG_TRAPEZOID: {
       if (n != 11 || !(c[0] > 0 && c[3] > 0 && c[4] > 0 && c[5] > 0 && c[7] > 0 && c[8] > 0 && c[9] > 0))
         return Fail(err, "trapezoid needs 11 parameters with positive lengths");
@@ -186,6 +236,14 @@ bool ZooExtent(int type, const double *b, double lo[3], double hi[3]) {
       const ParabP *q = reinterpret_cast<const ParabP *>(b);
       lo[0] = -q->dx; lo[1] = -q->dy; lo[2] = -q->dz;
       hi[0] = q->dx; hi[1] = q->dy; hi[2] = q->dz;
+      return true;
+    }
+    case VG_SPHERE: {
+      const SphereP *q = reinterpret_cast<const SphereP *>(b);
+      for (int i = 0; i < 3; ++i) {
+        lo[i] = -q->fRmax;
+        hi[i] = q->fRmax;
+      }
       return true;
     }
     case VG_TRAPEZOID: {

@@ -178,7 +178,42 @@ __global__ __launch_bounds__(kBlock) void ShapeQueryKernel(const typename S::P P
   }
 }
 
+// Plain FP64 form (the benchmark denominator): one thread per query, the
+// placement record and shape parameters read from global memory through the
+// placed-volume id (as the reference's device virtual call chases its object
+// pointers), the transform kind switched at run time, 8-byte loads.
+template <class S, int Q>
+__global__ __launch_bounds__(kBlock) void PlainQueryKernel(const PlacedRec *__restrict__ placed,
+                                                           const double *__restrict__ params, int32_t pid,
+                                                           const QArgs a) {
+  const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const PlacedRec &pr = placed[pid];
+  const typename S::P &P = *reinterpret_cast<const typename S::P *>(params + pr.param_off);
+  const V3 p{a.x[i], a.y[i], a.z[i]};
+  V3 d{0, 0, 0};
+  if constexpr (NeedsDir(Q)) d = V3{a.dx[i], a.dy[i], a.dz[i]};
+  double sm = kInfinity;
+  if constexpr (NeedsStep(Q)) sm = a.sm ? a.sm[i] : kInfinity;
+  if constexpr (Q == kQContains) {
+    reinterpret_cast<uint8_t *>(a.out)[i] = S::Contains(P, XfPointRt(pr.xf, pr.xfkind, p)) ? 1 : 0;
+  } else if constexpr (Q == kQInside) {
+    reinterpret_cast<int32_t *>(a.out)[i] = S::Inside(P, XfPointRt(pr.xf, pr.xfkind, p));
+  } else if constexpr (Q == kQDistIn) {
+    reinterpret_cast<double *>(a.out)[i] =
+        S::DistanceToIn(P, XfPointRt(pr.xf, pr.xfkind, p), XfDirRt(pr.xf, pr.xfkind, d), sm);
+  } else if constexpr (Q == kQDistOut) {
+    reinterpret_cast<double *>(a.out)[i] = S::DistanceToOut(P, p, d, sm);
+  } else if constexpr (Q == kQSafeIn) {
+    reinterpret_cast<double *>(a.out)[i] = S::SafetyToIn(P, XfPointRt(pr.xf, pr.xfkind, p));
+  } else if constexpr (Q == kQSafeOut) {
+    reinterpret_cast<double *>(a.out)[i] = S::SafetyToOut(P, p);
+  }
+}
+
 bool Aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+int g_variant = 0;  // 0 = tuned kernels, 1 = plain FP64 form
 
 template <class S, int XK, int Q>
 int LaunchQ(const typename S::P &P, const Xf &X, const QArgs &a, hipStream_t s) {
@@ -207,8 +242,34 @@ int LaunchXf(const typename S::P &P, const PlacedRec &pr, const QArgs &a, hipStr
   }
 }
 
+template <class S, int Q>
+int LaunchPlain(const vg_geometry *g, int32_t pid, const QArgs &a, hipStream_t s) {
+  const uint64_t blocks = (a.n + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL((PlainQueryKernel<S, Q>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g->d_placed, g->d_params,
+                     pid, a);
+  return CheckHip(hipGetLastError(), "plain shape query launch");
+}
+
 template <class S>
-int LaunchShape(const double *block, const PlacedRec &pr, int q, const QArgs &a, hipStream_t s) {
+int LaunchShapePlain(const vg_geometry *g, int32_t pid, int q, const QArgs &a, hipStream_t s) {
+  switch (q) {
+    case kQContains: return LaunchPlain<S, kQContains>(g, pid, a, s);
+    case kQInside: return LaunchPlain<S, kQInside>(g, pid, a, s);
+    case kQDistIn: return LaunchPlain<S, kQDistIn>(g, pid, a, s);
+    case kQDistOut: return LaunchPlain<S, kQDistOut>(g, pid, a, s);
+    case kQSafeIn: return LaunchPlain<S, kQSafeIn>(g, pid, a, s);
+    case kQSafeOut: return LaunchPlain<S, kQSafeOut>(g, pid, a, s);
+  }
+  return SetError(VG_ERR_UNSUPPORTED, "the plain form covers the six basic queries only");
+}
+
+template <class S>
+int LaunchShape(const vg_geometry *g, int32_t pid, const double *block, const PlacedRec &pr, int q, const QArgs &a,
+                hipStream_t s) {
+  if (g_variant == 1) {
+    if (a.n == 0) return VG_OK;
+    return LaunchShapePlain<S>(g, pid, q, a, s);
+  }
   typename S::P P;
   std::memcpy(&P, block, sizeof(P));
   switch (q) {
@@ -231,23 +292,30 @@ int LaunchShape(const double *block, const PlacedRec &pr, int q, const QArgs &a,
 bool ShapeHasKernels(int type) {
   switch (type) {
     case VG_BOX: case VG_TUBE: case VG_ORB: case VG_TRD: case VG_PARALLELEPIPED: case VG_PARABOLOID:
-    case VG_TRAPEZOID:
+    case VG_TRAPEZOID: case VG_SPHERE:
       return true;
   }
   return false;
+}
+
+int SetKernelVariant(int v) {
+  if (v != 0 && v != 1) return SetError(VG_ERR_INVALID, "kernel variant must be 0 (tuned) or 1 (plain FP64)");
+  g_variant = v;
+  return VG_OK;
 }
 
 int LaunchQuery(const vg_geometry *g, int32_t placed, int q, const QArgs &a, hipStream_t s) {
   const PlacedRec &pr = g->placed[placed];
   const double *block = g->params.data() + pr.param_off;
   switch (pr.type) {
-    case VG_BOX: return LaunchShape<BoxS>(block, pr, q, a, s);
-    case VG_TUBE: return LaunchShape<TubeS>(block, pr, q, a, s);
-    case VG_ORB: return LaunchShape<OrbS>(block, pr, q, a, s);
-    case VG_TRD: return LaunchShape<TrdS>(block, pr, q, a, s);
-    case VG_PARALLELEPIPED: return LaunchShape<ParaS>(block, pr, q, a, s);
-    case VG_PARABOLOID: return LaunchShape<ParabS>(block, pr, q, a, s);
-    case VG_TRAPEZOID: return LaunchShape<TrapS>(block, pr, q, a, s);
+    case VG_BOX: return LaunchShape<BoxS>(g, placed, block, pr, q, a, s);
+    case VG_TUBE: return LaunchShape<TubeS>(g, placed, block, pr, q, a, s);
+    case VG_ORB: return LaunchShape<OrbS>(g, placed, block, pr, q, a, s);
+    case VG_TRD: return LaunchShape<TrdS>(g, placed, block, pr, q, a, s);
+    case VG_PARALLELEPIPED: return LaunchShape<ParaS>(g, placed, block, pr, q, a, s);
+    case VG_PARABOLOID: return LaunchShape<ParabS>(g, placed, block, pr, q, a, s);
+    case VG_TRAPEZOID: return LaunchShape<TrapS>(g, placed, block, pr, q, a, s);
+    case VG_SPHERE: return LaunchShape<SphereS>(g, placed, block, pr, q, a, s);
   }
   return SetError(VG_ERR_UNSUPPORTED, "no device kernels for shape type " + std::to_string(pr.type));
 }
