@@ -217,6 +217,89 @@ int vg_safety_to_out(const vg_geometry *g, int32_t placed, const double *x, cons
 int vg_safety_to_out_minimize(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z,
                               size_t n, double *inout, void *stream);
 
+/* -------------------------------------------------------------- navigation */
+/*
+ * Navigation states of a basket of tracks, structure-of-arrays in device
+ * memory.  Replaces NavStatePool (navigation/NavStatePool.h:34-110), whose
+ * states are NavigationState objects (navigation/NavigationState.h) packed
+ * array-of-structures; here
+ *   level[i]                 number of volumes on the path of track i
+ *                            (0: outside the world)
+ *   path[l * capacity + i]   placed id at depth l (0 = world), -1 beyond
+ *                            level[i]
+ *   boundary[i]              1 if the state is on a boundary
+ * Any device buffers of these sizes may be used (torch tensors, for one);
+ * vg_nav_states_create allocates them.
+ */
+typedef struct vg_nav_states {
+  int32_t *level;
+  int32_t *path;
+  uint8_t *boundary;
+  uint64_t capacity;
+  int32_t depth; /* path rows; at least vg_geometry_max_depth() */
+  int32_t reserved;
+} vg_nav_states;
+
+/* Navigator selection (navigation/SimpleNavigator.h:341-482,
+ * navigation/ABBoxNavigator.h:201-424). */
+#define VG_NAV_ABBOX 0
+#define VG_NAV_SIMPLE 1
+
+/* Largest geometry depth the device navigator handles. */
+#define VG_NAV_MAX_DEPTH 16
+
+/* NavStatePool(size, depth): allocates the three arrays on the current
+ * device (synchronous).  depth <= 0 takes the geometry's depth. */
+int vg_nav_states_create(const vg_geometry *g, uint64_t capacity, int32_t depth, vg_nav_states *out);
+/* Frees arrays made by vg_nav_states_create. */
+int vg_nav_states_free(vg_nav_states *s);
+/* NavStatePool::CopyFromGpu / CopyToGpu (NavStatePool.h:62-63): copies the
+ * first n states to / from host arrays of the same layout, with row stride
+ * n on the host side (synchronous). */
+int vg_nav_states_to_host(const vg_nav_states *s, uint64_t n, int32_t *level, int32_t *path, uint8_t *boundary);
+int vg_nav_states_from_host(vg_nav_states *s, uint64_t n, const int32_t *level, const int32_t *path,
+                            const uint8_t *boundary);
+
+/* SimpleNavigator::LocatePoint(world, point, state, top=true)
+ * (navigation/SimpleNavigator.h:205-241) for n global points; the states
+ * are cleared first (boundary = 0). */
+int vg_locate_points(const vg_geometry *g, const double *x, const double *y, const double *z, size_t n,
+                     vg_nav_states *states, void *stream);
+
+/* FindNextBoundaryAndStep(globalpoint, globaldir, currentstate, newstate,
+ * pstep, step) of the selected navigator (ABBoxNavigator.h:201-424 /
+ * SimpleNavigator.h:341-482) for n tracks.  pstep may be NULL (kInfinity
+ * for all).  `cur` and `next` must not alias.  A state outside the world
+ * (level 0) stays outside with step 0; a state whose level exceeds
+ * VG_NAV_MAX_DEPTH or whose path holds an id that is not a placed volume is
+ * copied unchanged with step NaN. */
+int vg_find_next_boundary_and_step(const vg_geometry *g, int32_t navigator, const double *x, const double *y,
+                                   const double *z, const double *dx, const double *dy, const double *dz,
+                                   const double *pstep, size_t n, const vg_nav_states *cur, vg_nav_states *next,
+                                   double *step, void *stream);
+
+/* ABBoxNavigator::GetSafety (navigation/ABBoxNavigator.h:428-452); 0 for
+ * states outside the world, NaN for malformed states. */
+int vg_get_safety(const vg_geometry *g, const double *x, const double *y, const double *z, size_t n,
+                  const vg_nav_states *cur, double *safety, void *stream);
+
+/*
+ * The track loop of the reference's TraceTrack benchmark
+ * (test/globalbenchmarks/TraceTrack.cpp:170-220) run on the device, one
+ * track per lane: LocatePoint from the world, then FindNextBoundaryAndStep
+ * with `pstep` and p += d * (step + 1e-6) until the state leaves the world
+ * or max_steps steps were taken.  Per track: steps taken, final point,
+ * summed step length, and a 64-bit FNV-1a-style hash over the sequence of
+ * (new top placed id + 1) * 2 + boundary flag, starting from
+ * 1469598103934665603 with multiplier 1099511628211.  exact_steps (nullable)
+ * receives, per track, the number of ABBox steps that took the exact
+ * sorted-order path instead of the order-free one.
+ */
+int vg_trace_tracks(const vg_geometry *g, int32_t navigator, const double *x, const double *y, const double *z,
+                    const double *dx, const double *dy, const double *dz, size_t n, double pstep, int32_t max_steps,
+                    int32_t *nsteps, double *fx, double *fy, double *fz, double *travelled, uint64_t *hash,
+                    int32_t *exact_steps, void *stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -121,3 +121,85 @@ def reference(fast=False):
         path = os.path.join(HERE, "_ref", name)
         _cache[key] = Lib(path, "vgref_") if os.path.exists(path) else None
     return _cache[key]
+
+
+# ---------------------------------------------------------------- navigation
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class Nav:
+    """Flat geometry + navigator of one oracle flavour."""
+
+    def __init__(self, lib, flat):
+        self.lib = lib
+        f = lib.lib
+        pre = lib.prefix
+        self.pre = pre
+        getattr(f, pre + "geom_create").restype = ctypes.c_void_p
+        getattr(f, pre + "geom_create").argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_int] + \
+            [ctypes.c_void_p] * 3 + [ctypes.c_int] + [ctypes.c_void_p] * 2 + [ctypes.c_int]
+        getattr(f, pre + "geom_max_depth").argtypes = [ctypes.c_void_p]
+        getattr(f, pre + "locate").argtypes = [ctypes.c_void_p, ctypes.c_long] + [ctypes.c_void_p] * 3 + \
+            [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        getattr(f, pre + "nav_step").argtypes = [ctypes.c_void_p, ctypes.c_long] + [ctypes.c_void_p] * 6 + \
+            [ctypes.c_int] + [ctypes.c_void_p] * 6 + [ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        getattr(f, pre + "safety").argtypes = [ctypes.c_void_p, ctypes.c_long] + [ctypes.c_void_p] * 3 + \
+            [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        getattr(f, pre + "trace").argtypes = [ctypes.c_void_p, ctypes.c_long] + [ctypes.c_void_p] * 6 + \
+            [ctypes.c_double, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 6 + [ctypes.c_int]
+        self.flat = flat
+        self.h = getattr(f, pre + "geom_create")(
+            len(flat.shape_type), _ptr(flat.shape_type), _ptr(flat.shape_param_begin), _ptr(flat.params),
+            len(flat.logical_shape), _ptr(flat.logical_shape), _ptr(flat.logical_daughter_begin), _ptr(flat.daughters),
+            len(flat.placed_logical), _ptr(flat.placed_logical), _ptr(flat.placed_transform), flat.world)
+        if not self.h:
+            raise ValueError(pre + "geom_create failed")
+        self.depth = getattr(f, pre + "geom_max_depth")(self.h)
+
+    def _f(self, n):
+        return getattr(self.lib.lib, self.pre + n)
+
+    def locate(self, p, nthreads=1):
+        p = np.ascontiguousarray(p, np.float64)
+        n = p.shape[1]
+        level = np.zeros(n, np.int32)
+        path = np.full((self.depth, n), -1, np.int32)
+        self._f("locate")(self.h, n, _ptr(p[0]), _ptr(p[1]), _ptr(p[2]), self.depth, _ptr(level), _ptr(path), nthreads)
+        return level, path
+
+    def step(self, p, d, level, path, pstep=np.inf, kind=0, nthreads=1):
+        p = np.ascontiguousarray(p, np.float64)
+        d = np.ascontiguousarray(d, np.float64)
+        n = p.shape[1]
+        level = np.ascontiguousarray(level, np.int32)
+        path = np.ascontiguousarray(path, np.int32)
+        lo = np.zeros(n, np.int32)
+        po = np.full((self.depth, n), -1, np.int32)
+        bo = np.zeros(n, np.uint8)
+        so = np.zeros(n, np.float64)
+        self._f("nav_step")(self.h, n, _ptr(p[0]), _ptr(p[1]), _ptr(p[2]), _ptr(d[0]), _ptr(d[1]), _ptr(d[2]),
+                            self.depth, _ptr(level), _ptr(path), _ptr(lo), _ptr(po), _ptr(bo), _ptr(so), float(pstep),
+                            int(kind), nthreads)
+        return lo, po, bo, so
+
+    def safety(self, p, level, path, nthreads=1):
+        p = np.ascontiguousarray(p, np.float64)
+        n = p.shape[1]
+        out = np.zeros(n)
+        self._f("safety")(self.h, n, _ptr(p[0]), _ptr(p[1]), _ptr(p[2]), self.depth,
+                          _ptr(np.ascontiguousarray(level, np.int32)), _ptr(np.ascontiguousarray(path, np.int32)),
+                          _ptr(out), nthreads)
+        return out
+
+    def trace(self, p, d, pstep=np.inf, kind=0, max_steps=100000, nthreads=1):
+        p = np.ascontiguousarray(p, np.float64)
+        d = np.ascontiguousarray(d, np.float64)
+        n = p.shape[1]
+        ns = np.zeros(n, np.int32)
+        fp = np.zeros((3, n))
+        tr = np.zeros(n)
+        hs = np.zeros(n, np.uint64)
+        self._f("trace")(self.h, n, _ptr(p[0]), _ptr(p[1]), _ptr(p[2]), _ptr(d[0]), _ptr(d[1]), _ptr(d[2]),
+                         float(pstep), int(kind), int(max_steps), _ptr(ns), _ptr(fp[0]), _ptr(fp[1]), _ptr(fp[2]),
+                         _ptr(tr), _ptr(hs), nthreads)
+        return ns, fp, tr, hs

@@ -103,6 +103,8 @@ Vector3D<Precision> V(const double *x, const double *y, const double *z, long i)
 
 }  // namespace
 
+VUnplacedVolume *RefMakeUnplaced(int type, const double *p, int n) { return MakeUnplaced(type, p, n); }
+
 extern "C" {
 
 void *vgref_placed_create(int type, const double *params, int nparams, const double *xf12) {

@@ -34,7 +34,7 @@ struct BoxS {
   }
   static VGA_HD bool Contains(const P &b, V3 p) {
     // Branch-free form of the ForInside=false kernel: identical result.
-    return !((Abs(p.x) > b.d[0]) | (Abs(p.y) > b.d[1]) | (Abs(p.z) > b.d[2]));
+    return !(int(Abs(p.x) > b.d[0]) | int(Abs(p.y) > b.d[1]) | int(Abs(p.z) > b.d[2]));
   }
   static VGA_HD int Inside(const P &b, V3 p) {
     bool cin = false, cout;

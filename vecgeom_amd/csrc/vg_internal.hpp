@@ -54,6 +54,7 @@ struct vg_geometry {
   vga::PlacedRec *d_placed = nullptr;
   int32_t *d_daughters = nullptr;
   float *d_aabb = nullptr;  // 6 floats per placed (ABBoxManager keeps floats)
+  int32_t nav_bad_type = 0;  // first shape type in the tree without device kernels (0: none)
 };
 
 namespace vga {
@@ -61,7 +62,11 @@ namespace vga {
 // Error handling (vg_capi.cpp).
 int SetError(int code, const std::string &msg);
 int CheckHip(hipError_t e, const char *what);
+const char *LastError();
 int EnsureDevice();
+// Validates and flattens a geometry on the host (vg_geometry.cpp); fills
+// every field but the device copies.  Returns a status.
+int BuildGeometry(const vg_flat_geometry *f, vg_geometry *g);
 
 // Builds the parameter block of one shape from its constructor parameters.
 // Returns an empty vector and sets *err on invalid input.
@@ -99,6 +104,20 @@ int LaunchQuery(const vg_geometry *g, int32_t placed, int q, const QArgs &a, hip
 bool ShapeHasKernels(int type);
 int SetKernelVariant(int v);
 
+}  // namespace vga
+
+namespace vga {
+// Navigation launches (vg_nav.hip); arguments validated by the caller.
+int NavLocate(const vg_geometry *g, const double *x, const double *y, const double *z, uint64_t n,
+              const vg_nav_states &s, hipStream_t st);
+int NavStep(const vg_geometry *g, int kind, const double *x, const double *y, const double *z, const double *dx,
+            const double *dy, const double *dz, const double *pstep, uint64_t n, const vg_nav_states &cur,
+            const vg_nav_states &nxt, double *step, hipStream_t st);
+int NavSafety(const vg_geometry *g, const double *x, const double *y, const double *z, uint64_t n,
+              const vg_nav_states &s, double *out, hipStream_t st);
+int NavTrace(const vg_geometry *g, int kind, const double *x, const double *y, const double *z, const double *dx,
+             const double *dy, const double *dz, uint64_t n, double pstep, int max_steps, int32_t *nsteps, double *fx,
+             double *fy, double *fz, double *travelled, uint64_t *hash, int32_t *exact, hipStream_t st);
 }  // namespace vga
 
 namespace vga {

@@ -289,15 +289,6 @@ int LaunchShape(const vg_geometry *g, int32_t pid, const double *block, const Pl
 
 }  // namespace
 
-bool ShapeHasKernels(int type) {
-  switch (type) {
-    case VG_BOX: case VG_TUBE: case VG_ORB: case VG_TRD: case VG_PARALLELEPIPED: case VG_PARABOLOID:
-    case VG_TRAPEZOID: case VG_SPHERE:
-      return true;
-  }
-  return false;
-}
-
 int SetKernelVariant(int v) {
   if (v != 0 && v != 1) return SetError(VG_ERR_INVALID, "kernel variant must be 0 (tuned) or 1 (plain FP64)");
   g_variant = v;

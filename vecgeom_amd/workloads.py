@@ -147,3 +147,124 @@ def shape_inputs(extent, xf, n, seed, surface_fraction=0.0, surface_sampler=None
         "lp": master_to_local(xf, p),
         "ld": master_to_local(xf, d, direction=True),
     }
+
+
+# ---------------------------------------------------------------- navigation
+# Nested benchmark geometry.  The nominal description (world box
+# 2000x2000x4000 mm, 64 tube shells with rmin 100..900 mm and dz 1800 mm,
+# 32 boxes of 20x40x200 mm per shell, 8 Trds per box) cannot be realised
+# without overlaps: 64 disjoint shells between r=100 and r=900 leave at most
+# 12.7 mm of radial thickness, and 32 boxes 40 mm wide do not fit around the
+# innermost shells.  The realisation used here, recorded so that every run
+# builds the same tree:
+#   world   box half-lengths (1000, 1000, 2000)
+#   shell k tube rmin_k = 100 + 800 k / 63, rmax_k = rmin_k + 12,
+#           dz = 1800, full phi, identity placement (k = 0..63)
+#   box     half-lengths (4, 20, 100): thin side radial, 40 mm tangential,
+#           200 mm along z; 32 per shell = 8 azimuths x 4 z-slices
+#           (z = -600, -200, 200, 600); azimuth a*45 deg + a random offset
+#           per shell; centre radius rmin_k + 6; rotated about z by its
+#           azimuth (a "rotated box")
+#   trd     dx1 3, dx2 2, dy1 15, dy2 10, dz 12; 8 per box along z at
+#           z = -87.5 + 25 j, translation only
+# All boxes share one logical volume and so do the Trds, as placements of
+# one logical volume do in the reference.
+NESTED = {"world": (1000.0, 1000.0, 2000.0), "shells": 64, "rmin0": 100.0, "rmin1": 900.0, "thick": 12.0,
+          "dz": 1800.0, "azimuths": 8, "zslices": (-600.0, -200.0, 200.0, 600.0), "box": (4.0, 20.0, 100.0),
+          "trd": (3.0, 2.0, 15.0, 10.0, 12.0), "trds": 8}
+
+
+def nested_geometry(seed=SEED_BASE + 4, shells=None):
+    """Builds the nested geometry; returns (GeoManager, flat arrays)."""
+    nshell = NESTED["shells"] if shells is None else shells
+    rng = np.random.Generator(np.random.Philox(seed))
+    gm = G.GeoManager()
+    world_lv = G.LogicalVolume("world", G.UnplacedBox(*NESTED["world"]), gm)
+    box_lv = G.LogicalVolume("box", G.UnplacedBox(*NESTED["box"]), gm)
+    trd_lv = G.LogicalVolume("trd", G.UnplacedTrd(*NESTED["trd"]), gm)
+    for j in range(NESTED["trds"]):
+        box_lv.PlaceDaughter(f"trd{j}", trd_lv, G.Transformation3D(0.0, 0.0, -87.5 + 25.0 * j))
+    na = NESTED["azimuths"]
+    for k in range(nshell):
+        rmin = NESTED["rmin0"] + (NESTED["rmin1"] - NESTED["rmin0"]) * k / 63.0
+        rmax = rmin + NESTED["thick"]
+        shell_lv = G.LogicalVolume(f"shell{k}", G.UnplacedTube(rmin, rmax, NESTED["dz"], 0.0, TWO_PI), gm)
+        phi0 = rng.uniform(0.0, TWO_PI / na)
+        rc = rmin + 0.5 * NESTED["thick"]
+        for b, zc in enumerate(NESTED["zslices"]):
+            for a in range(na):
+                phi = phi0 + a * TWO_PI / na
+                c, s = math.cos(phi), math.sin(phi)
+                xf = G.Transformation3D(rc * c, rc * s, zc, c, -s, 0.0, s, c, 0.0, 0.0, 0.0, 1.0)
+                shell_lv.PlaceDaughter(f"box{k}_{b}_{a}", box_lv, xf)
+        world_lv.PlaceDaughter(f"shell{k}", shell_lv, G.Transformation3D())
+    world = world_lv.Place("world")
+    gm.SetWorld(world)
+    return gm, gm.CloseGeometry()
+
+
+def track_starts(n, seed=SEED_BASE + 5, sigma=5.0):
+    """Vertices N(0, sigma^2) per axis and isotropic unit directions."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    p = rng.normal(0.0, sigma, size=(3, n))
+    d = isotropic(n, rng)
+    return p, d
+
+
+# Shapes placed as volumes in the zoo navigation geometry, with a point of
+# each shape's interior (local frame) that holds an Orb r=3 daughter.
+NAV_ZOO = [
+    ("Box", (0.0, 0.0, 0.0)),
+    ("Orb", (10.0, 0.0, 0.0)),
+    ("Sphere", (30.0 * math.cos(1.0), 30.0 * math.sin(1.0), 0.0)),
+    ("Tube", (0.0, 30.0, 0.0)),
+    ("Trd", (0.0, 0.0, 5.0)),
+    ("Trapezoid", (0.0, 0.0, 0.0)),
+    ("Parallelepiped", (0.0, 0.0, 0.0)),
+    ("Paraboloid", (0.0, 0.0, 0.0)),
+]
+
+
+def nav_zoo_geometry(seed=SEED_BASE + 6, names=None):
+    """World box of half-length 400 mm holding one placement of each shape
+    in NAV_ZOO on a 3x3 grid (pitch 220 mm, z = 0) with a random rotation;
+    each holds an Orb r=3 daughter at its listed interior point.  Returns
+    (GeoManager, flat arrays, list of (name, placed id))."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    shapes = dict(zoo_shapes())
+    names = [n for n, _ in NAV_ZOO] if names is None else names
+    inner = dict(NAV_ZOO)
+    gm = G.GeoManager()
+    world_lv = G.LogicalVolume("world", G.UnplacedBox(400.0, 400.0, 400.0), gm)
+    orb_lv = G.LogicalVolume("orb", G.UnplacedOrb(3.0), gm)
+    slots = [(x, y) for x in (-220.0, 0.0, 220.0) for y in (-220.0, 0.0, 220.0)]
+    placed = []
+    for k, name in enumerate(names):
+        lv = G.LogicalVolume(name, shapes[name], gm)
+        lv.PlaceDaughter(name + "_orb", orb_lv, G.Transformation3D(*inner[name]))
+        r = random_rotation(rng)
+        x, y = slots[k]
+        pv = world_lv.PlaceDaughter(name, lv, G.Transformation3D(x, y, 0.0, *r))
+        placed.append((name, pv))
+    gm.SetWorld(world_lv.Place("world"))
+    flat = gm.CloseGeometry()
+    return gm, flat, [(n, pv.id) for n, pv in placed]
+
+
+def nav_zoo_tracks(n, centres, seed=SEED_BASE + 7):
+    """A third of the tracks start uniform in |x|,|y|,|z| < 380 with
+    isotropic directions, a third start there aimed at a random shape
+    centre (jittered by 20 mm), a third start at a random shape centre
+    jittered by 10 mm with isotropic directions."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    p = rng.uniform(-380.0, 380.0, size=(3, n))
+    d = isotropic(n, rng)
+    group = rng.integers(0, 3, n)
+    cs = np.asarray(centres, dtype=np.float64)
+    c = cs[rng.integers(0, len(cs), n)].T + rng.normal(0.0, 20.0, (3, n))
+    v = c - p
+    v /= np.sqrt((v * v).sum(axis=0))
+    d[:, group == 1] = v[:, group == 1]
+    c2 = cs[rng.integers(0, len(cs), n)].T + rng.normal(0.0, 10.0, (3, n))
+    p[:, group == 2] = c2[:, group == 2]
+    return p, d
