@@ -30,6 +30,10 @@ HBM_PEAK_GBS = 8000.0
 BYTES = {"contains": 25, "inside": 28, "dist_in": 64, "dist_out": 64, "safe_in": 32, "safe_out": 32}
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -39,6 +43,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-FP64-form denominator")
+    ap.add_argument("--nav-tracks", type=int, default=1 << 20, help="tracks of the nested-geometry navigation run")
+    ap.add_argument("--nav-steps", type=int, default=3, help="timed trace launches")
+    ap.add_argument("--no-nav", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a device-to-device copy of MIB MiB (for HBM counter scaling)")
     return ap.parse_args()
@@ -142,12 +149,17 @@ def time_steps(launches, steps, stream, watch=None):
 
 
 def cpu_baseline(wl, target_s):
-    """Oracle (C++ restatement of the reference Scalar kernels) on the host:
-    the same queries on a bounded, evenly strided sample of the same inputs."""
+    """The reference's CPU basket path on the host: its SOA3D basket
+    virtuals, built from the reference's sources with its Release flags
+    (oracle/_ref/libvgref_fast.so), over contiguous ranges on every host core;
+    the oracle restatement stands in where that build is absent.  The same
+    queries on a bounded, evenly strided sample of the same inputs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
-    o = po.oracle()
+    ref = po.reference(fast=True)
+    o = ref if ref is not None and ref.has("basket") else po.oracle()
+    kind = "reference" if o is ref else "port"
     nthreads = max(1, min(16, os.cpu_count() or 1))
     # Calibrate: one shape, all queries, small sample.
     frac = 1.0 / 64
@@ -159,18 +171,19 @@ def cpu_baseline(wl, target_s):
         t0 = time.perf_counter()
         for it, (name, u), xf in [z for _ in range(reps) for z in zip(wl.items, wl.shapes, wl.xfs)]:
             h = o.placed(u.type, u.params, xf.array())
+            run = h.basket if kind == "reference" else h.query
             hp = it["host"]
             step = int(round(1 / frac))
             sel_all = slice(0, wl.n, step)
             iout = it["outside_idx"][::step]
             iin = it["inside_idx"][::step]
             P, D, LP, LD = hp["p"], hp["d"], hp["lp"], hp["ld"]
-            h.query(po.Q_CONTAINS, P[:, sel_all], nthreads=nthreads)
-            h.query(po.Q_INSIDE, P[:, sel_all], nthreads=nthreads)
-            h.query(po.Q_DIST_IN, P[:, iout], D[:, iout], np.full(len(iout), 1e30), nthreads=nthreads)
-            h.query(po.Q_SAFE_IN, P[:, iout], nthreads=nthreads)
-            h.query(po.Q_DIST_OUT, LP[:, iin], LD[:, iin], np.full(len(iin), 1e30), nthreads=nthreads)
-            h.query(po.Q_SAFE_OUT, LP[:, iin], nthreads=nthreads)
+            run(po.Q_CONTAINS, P[:, sel_all], nthreads=nthreads)
+            run(po.Q_INSIDE, P[:, sel_all], nthreads=nthreads)
+            run(po.Q_DIST_IN, P[:, iout], D[:, iout], np.full(len(iout), 1e30), nthreads=nthreads)
+            run(po.Q_SAFE_IN, P[:, iout], nthreads=nthreads)
+            run(po.Q_DIST_OUT, LP[:, iin], LD[:, iin], np.full(len(iin), 1e30), nthreads=nthreads)
+            run(po.Q_SAFE_OUT, LP[:, iin], nthreads=nthreads)
             total_q += 2 * len(range(0, wl.n, step)) + 2 * len(iout) + 2 * len(iin)
         dt = time.perf_counter() - t0
         if dt >= 0.5 * target_s:
@@ -183,10 +196,109 @@ def cpu_baseline(wl, target_s):
         "value": total_q / dt / 1e9,
         "unit": "Gqueries/s",
         "cores": nthreads,
-        "kind": "port",
+        "kind": kind,
         "sample": f"every {int(round(1 / frac))}th query of the same workload x{reps} ({total_q} queries, {dt:.1f} s), "
-                  f"oracle C++ restatement of the reference Scalar backend, std::thread over contiguous ranges",
+                  + ("reference SOA3D basket virtuals, Scalar backend, Release flags (-O3 -ffast-math)"
+                     if kind == "reference" else "oracle C++ restatement of the reference Scalar backend")
+                  + ", std::thread over contiguous ranges",
     }
+
+
+def nav_bench(args, dev, stream, with_cpu):
+    """Navigation on the nested geometry: 2^20 tracks from N(0, 5 mm) vertices
+    with isotropic directions, ABBoxNavigator steps with pstep = kInfinity
+    until every track has left the world (the TraceTrack loop).  The fused
+    form runs the whole loop on the device (one launch); the plain form
+    launches one FindNextBoundaryAndStep per step over the whole basket from
+    the host, as the reference's GPU navigation benchmark does, and updates
+    positions between launches."""
+    import torch
+
+    from vecgeom_amd import geometry as G
+    from vecgeom_amd import nav as N
+    from vecgeom_amd import workloads as W
+
+    _, flat = W.nested_geometry()
+    geom = G.DeviceGeometry(flat)
+    n = args.nav_tracks
+    p0, d0 = W.track_starts(n)
+    t = lambda a: tuple(torch.from_numpy(np.ascontiguousarray(a[i])).to(dev) for i in range(3))
+    p, d = t(p0), t(d0)
+    out = N.TraceResult(n, dev)
+    N.trace_tracks(geom, p, d, navigator=N.ABBOX, out=out)
+    torch.cuda.synchronize()
+    total_steps = int(out.nsteps.to(torch.int64).sum().item())
+    exact = int(out.exact.to(torch.int64).sum().item())
+    ms = []
+    for _ in range(args.nav_steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        N.trace_tracks(geom, p, d, navigator=N.ABBOX, out=out)
+        e1.record(stream)
+        ms.append((e0, e1))
+    torch.cuda.synchronize()
+    ms = [a.elapsed_time(b) for a, b in ms]
+    fused_ms = float(np.mean(ms))
+    res = {"workload": f"nested geometry (64 tube shells x 32 boxes x 8 Trds), {n} tracks from N(0,5mm) vertices, "
+                       "isotropic, ABBoxNavigator, pstep=inf, until exit",
+           "tracks": n, "steps": total_steps, "mean_steps_per_track": round(total_steps / n, 3),
+           "exact_walk_steps": exact, "kernel": "TraceKernel (fused TraceTrack loop)",
+           "avg_launch_ms": round(fused_ms, 3), "value": round(total_steps / (fused_ms * 1e-3) / 1e9, 4),
+           "unit": "G steps/s", "dtype": "f64"}
+    # Plain form: one launch per step from the host.
+    cur = N.locate_points(geom, p)
+    new = N.NavStates(geom, n, device=dev)
+    step = torch.empty(n, dtype=torch.float64, device=dev)
+    q = [a.clone() for a in p]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    launches = 0
+    for _ in range(100000):
+        act = cur.level > 0
+        if not bool(act.any()):
+            break
+        N.find_next_boundary_and_step(geom, tuple(q), d, cur, new=new, step=step)
+        s = step + 1e-6
+        for i in range(3):
+            q[i] = torch.where(act, q[i] + d[i] * s, q[i])
+        cur, new = new, cur
+        launches += 1
+    torch.cuda.synchronize()
+    pel = time.perf_counter() - t0
+    res["plain_per_step_launch"] = {"value": round(total_steps / pel / 1e9, 4), "unit": "G steps/s",
+                                    "launches": launches, "seconds": round(pel, 4),
+                                    "speedup_of_fused": round((total_steps / (fused_ms * 1e-3)) / (total_steps / pel), 3)}
+    if with_cpu:
+        res["cpu_baseline"] = nav_cpu_baseline(flat, p0, d0, args.cpu_seconds / 2)
+    geom.free()
+    return res
+
+
+def nav_cpu_baseline(flat, p0, d0, target_s):
+    """The reference's own ABBoxNavigator (built from its sources with its
+    Release flags) on one core, on a bounded prefix of the same tracks; the
+    navigator keeps a process-wide hit list, so it runs on one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    ref = po.reference(fast=True)
+    kind, lib, cores = "reference", ref, 1
+    if ref is None:
+        kind, lib, cores = "port", po.oracle(), max(1, min(16, os.cpu_count() or 1))
+    nav = po.Nav(lib, flat)
+    m = 256
+    while True:
+        t0 = time.perf_counter()
+        ns, _, _, _ = nav.trace(p0[:, :m], d0[:, :m], kind=0, nthreads=cores)
+        dt = time.perf_counter() - t0
+        if dt >= 0.5 * target_s or m >= p0.shape[1]:
+            break
+        m = min(p0.shape[1], int(m * max(2.0, target_s / max(dt, 1e-3))))
+    steps = int(ns.sum())
+    return {"value": round(steps / dt / 1e9, 6), "unit": "G steps/s", "cores": cores, "kind": kind,
+            "sample": f"first {m} tracks of the same workload ({steps} steps, {dt:.1f} s), "
+                      + ("reference ABBoxNavigator, Release flags (-O3 -ffast-math)" if kind == "reference"
+                         else "oracle C++ restatement")}
 
 
 def main():
@@ -220,6 +332,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    log(f"workload ready: {len(wl.shapes)} shapes x {args.points} points")
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -252,6 +365,7 @@ def main():
             "kernel": f"ShapeQueryKernel<{dom['shape']},{dom['query']}>", "avg_launch_us": round(1e3 * dom_ms, 3),
             "bytes_per_query": BYTES[dom["query"]], "queries_per_launch": dom["n"], "bytes_per_launch": dom_bytes}
 
+    log(f"timed shape steps done: {value:.3f} Gqueries/s")
     plain = None
     if not args.no_plain:
         # The plain FP64 GPU form: same queries, one thread per query with
@@ -268,7 +382,13 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        log("cpu baseline (shapes)")
         cpu = cpu_baseline(wl, args.cpu_seconds)
+
+    nav = None
+    if not args.no_nav:
+        log("navigation")
+        nav = nav_bench(args, dev, stream, not args.no_cpu)
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gqueries/s", "n_gpus": 1, "steps": args.steps,
@@ -281,6 +401,7 @@ def main():
         "roofline": roof,
         "plain_fp64_gpu": plain,
         "cpu_baseline": cpu,
+        "nav": nav,
         "per_kernel": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()} for r in rows],
     }
     print(json.dumps(line), flush=True)

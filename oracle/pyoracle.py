@@ -47,6 +47,9 @@ class Lib:
         f("query").restype = ctypes.c_int
         f("query").argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_long] + [ctypes.c_void_p] * 8 + [ctypes.c_int]
         f("shape_extent").argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        if self.has("basket"):
+            f("basket").restype = ctypes.c_int
+            f("basket").argtypes = f("query").argtypes
 
     def _f(self, name):
         return getattr(self.lib, self.prefix + name)
@@ -89,6 +92,24 @@ class Placed:
                                   int(nthreads))
         if rc != 0:
             raise RuntimeError(f"{self.lib.prefix}query failed rc={rc}")
+        return out
+
+    def basket(self, q, p, d=None, step_max=None, nthreads=1):
+        """The reference's SOA3D basket virtuals (reference shim only);
+        queries 0..5 as in query()."""
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        n = p.shape[1]
+        out = np.empty(n, dtype=QUERY_DTYPE[q])
+        if d is not None:
+            d = np.ascontiguousarray(d, dtype=np.float64)
+            dx, dy, dz = _ptr(d[0]), _ptr(d[1]), _ptr(d[2])
+        else:
+            dx = dy = dz = None
+        sm = None if step_max is None else np.ascontiguousarray(step_max, dtype=np.float64)
+        rc = self.lib._f("basket")(self.h, q, n, _ptr(p[0]), _ptr(p[1]), _ptr(p[2]), dx, dy, dz, _ptr(sm), _ptr(out),
+                                   int(nthreads))
+        if rc != 0:
+            raise RuntimeError(f"{self.lib.prefix}basket failed rc={rc}")
         return out
 
     def __del__(self):

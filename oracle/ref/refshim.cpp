@@ -10,10 +10,12 @@
 #include <cstdint>
 #include <cstring>
 #include <cmath>
+#include <thread>
 #include <vector>
 
 #include "base/Global.h"
 #include "base/Transformation3D.h"
+#include "base/SOA3D.h"
 #include "base/Vector3D.h"
 #include "volumes/LogicalVolume.h"
 #include "volumes/PlacedVolume.h"
@@ -144,6 +146,57 @@ int vgref_query(void *h, int query, long n, const double *px, const double *py, 
       default: return -2;
     }
   }
+  return 0;
+}
+
+// The reference's own SOA3D basket virtuals (volumes/PlacedVolume.h:147-246,
+// basket loops volumes/ShapeImplementationHelper.h:588-748), each thread on a
+// contiguous range through non-owning SOA3D views (base/SOA3D.h:173-176).
+// Queries as in vgref_query (0..5); Contains writes uint8 through a bool
+// buffer.  Used to time the reference's CPU path.
+int vgref_basket(void *h, int query, long n, const double *px, const double *py, const double *pz, const double *dx,
+                 const double *dy, const double *dz, const double *step_max, void *out, int nthreads) {
+  auto *r = static_cast<RefPlaced *>(h);
+  if (!r || query < 0 || query > 5) return -1;
+  VPlacedVolume const *pv = r->placed;
+  auto run = [&](long b, long e) {
+    const size_t m = (size_t)(e - b);
+    SOA3D<Precision> P(const_cast<double *>(px + b), const_cast<double *>(py + b), const_cast<double *>(pz + b), m);
+    switch (query) {
+      case 0: {
+        std::vector<char> tmp(m);
+        bool *o = reinterpret_cast<bool *>(tmp.data());
+        pv->Contains(P, o);
+        for (size_t i = 0; i < m; ++i) static_cast<uint8_t *>(out)[b + i] = o[i] ? 1 : 0;
+        break;
+      }
+      case 1: pv->Inside(P, static_cast<Inside_t *>(out) + b); break;
+      case 2: {
+        SOA3D<Precision> D(const_cast<double *>(dx + b), const_cast<double *>(dy + b), const_cast<double *>(dz + b), m);
+        pv->DistanceToIn(P, D, step_max + b, static_cast<double *>(out) + b);
+        break;
+      }
+      case 3: {
+        SOA3D<Precision> D(const_cast<double *>(dx + b), const_cast<double *>(dy + b), const_cast<double *>(dz + b), m);
+        pv->DistanceToOut(P, D, step_max + b, static_cast<double *>(out) + b);
+        break;
+      }
+      case 4: pv->SafetyToIn(P, static_cast<double *>(out) + b); break;
+      case 5: pv->SafetyToOut(P, static_cast<double *>(out) + b); break;
+    }
+  };
+  if (nthreads <= 1 || n < 1024) {
+    run(0, n);
+    return 0;
+  }
+  std::vector<std::thread> th;
+  const long chunk = (n + nthreads - 1) / nthreads;
+  for (int t = 0; t < nthreads; ++t) {
+    const long b = t * chunk, e = std::min(n, b + chunk);
+    if (b >= e) break;
+    th.emplace_back(run, b, e);
+  }
+  for (auto &x : th) x.join();
   return 0;
 }
 
