@@ -77,6 +77,14 @@ inline V3 operator*(V3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
 // base/Vector3D.h Dot: x*x' + y*y' + z*z' evaluated left to right.
 inline double Dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
+// ATan2 follows backend/scalar/Backend.h:130-137.
+inline double ATan2(double y, double x) {
+  if (x != 0) return std::atan2(y, x);
+  if (y > 0) return kPi / 2;
+  if (y < 0) return -kPi / 2;
+  return 0;
+}
+
 // Transformation3D restated with the reference's operation order.
 struct Transform {
   double t[3] = {0, 0, 0};

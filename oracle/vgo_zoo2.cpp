@@ -3,17 +3,10 @@
 #include "vgo.h"
 
 namespace vgo {
-Shape *MakeCone(const double *, int) { return nullptr; }
+
 // ---------------------------------------------------------------- Sphere
 // volumes/kernel/SphereImplementation.h:257-1520, volumes/UnplacedSphere.h:
 // 63-215 (angle checks, cached trigonometry), source/UnplacedSphere.cpp:25-95.
-// ATan2 follows backend/scalar/Backend.h:130-137.
-inline double ATan2(double y, double x) {
-  if (x != 0) return std::atan2(y, x);
-  if (y > 0) return kPi / 2;
-  if (y < 0) return -kPi / 2;
-  return 0;
-}
 
 struct Sphere : Shape {
   double fRmin, fRmax, fSPhi = 0, fDPhi = 0, fSTheta = 0, fDTheta = 0;

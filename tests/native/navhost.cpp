@@ -186,6 +186,39 @@ void navhost_safety(void *hv, long n, const double *x, const double *y, const do
     Safety<16>(h, n, x, y, z, level_in, path_in, out, nthreads);
 }
 
+// The six queries of placed volume `pv` through the device shape code
+// (frames as the C-ABI: mother frame for Contains, Inside, DistanceToIn,
+// SafetyToIn; local frame for DistanceToOut, SafetyToOut).  query 0..5 as the
+// oracle's; out is uint8 / int32 / double.
+int navhost_query(void *hv, int pv, int query, long n, const double *x, const double *y, const double *z,
+                  const double *dx, const double *dy, const double *dz, const double *sm, void *out) {
+  auto *h = static_cast<HostGeom *>(hv);
+  const NavDev &g = h->dev;
+  const PlacedRec &r = g.Pv(pv);
+  Xf xf;
+  LoadXf(r, xf);
+  const double *blk = g.params + r.param_off;
+  for (long i = 0; i < n; ++i) {
+    const V3 p{x[i], y[i], z[i]};
+    const double s = sm ? sm[i] : kInfinity;
+    switch (query) {
+      case 0: static_cast<uint8_t *>(out)[i] = DContains(r.type, blk, XfPointRt(xf, r.xfkind, p)) ? 1 : 0; break;
+      case 1: static_cast<int32_t *>(out)[i] = DInside(r.type, blk, XfPointRt(xf, r.xfkind, p)); break;
+      case 2: {
+        const V3 d{dx[i], dy[i], dz[i]};
+        static_cast<double *>(out)[i] =
+            DDistIn(r.type, blk, XfPointRt(xf, r.xfkind, p), XfDirRt(xf, r.xfkind, d), s);
+        break;
+      }
+      case 3: static_cast<double *>(out)[i] = DDistOut(r.type, blk, p, V3{dx[i], dy[i], dz[i]}, s); break;
+      case 4: static_cast<double *>(out)[i] = DSafeIn(r.type, blk, XfPointRt(xf, r.xfkind, p)); break;
+      case 5: static_cast<double *>(out)[i] = DSafeOut(r.type, blk, p); break;
+      default: return -1;
+    }
+  }
+  return 0;
+}
+
 void navhost_trace(void *hv, long n, const double *x, const double *y, const double *z, const double *dx,
                    const double *dy, const double *dz, double pstep, int kind, int max_steps, int *nsteps, double *fx,
                    double *fy, double *fz, double *travelled, unsigned long long *hash, int nthreads) {

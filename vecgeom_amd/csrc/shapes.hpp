@@ -4,3 +4,4 @@
 #include "shapes_simple.hpp"
 #include "shapes_trap.hpp"
 #include "shapes_sphere.hpp"
+#include "shapes_cone.hpp"

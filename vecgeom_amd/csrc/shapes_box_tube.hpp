@@ -96,6 +96,9 @@ struct WedgeP {
   double n1x, n1y, n2x, n2y;  // inward normals
 };
 
+// Host-side construction of a wedge (vg_params.cpp).
+WedgeP MakeWedge(double angle, double zero);
+
 struct WedgeS {
   static VGA_HD bool OnSurface(double ax, double ay, double nx, double ny, V3 p) {
     if (!(ax * p.x + ay * p.y >= 0.)) return false;

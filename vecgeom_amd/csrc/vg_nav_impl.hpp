@@ -90,6 +90,10 @@ struct NavDev {
       using S = SphereS;                       \
       BODY;                                    \
     }                                          \
+    case VG_CONE: {                            \
+      using S = ConeS;                         \
+      BODY;                                    \
+    }                                          \
   }
 
 #define VGA_P(blk) (*reinterpret_cast<const typename S::P *>(blk))
@@ -97,6 +101,10 @@ struct NavDev {
 VGA_NAV bool DContains(int type, const double *blk, V3 p) {
   VGA_SHAPE_CASES(type, return S::Contains(VGA_P(blk), p));
   return false;
+}
+VGA_NAV int DInside(int type, const double *blk, V3 p) {
+  VGA_SHAPE_CASES(type, return S::Inside(VGA_P(blk), p));
+  return kOutside;
 }
 VGA_NAV double DDistIn(int type, const double *blk, V3 p, V3 d, double sm) {
   VGA_SHAPE_CASES(type, return S::DistanceToIn(VGA_P(blk), p, d, sm));

@@ -25,20 +25,6 @@ bool Fail(std::string *err, const std::string &m) {
   return false;
 }
 
-// source/Wedge.cpp:16-43
-WedgeP MakeWedge(double angle, double zero) {
-  WedgeP w;
-  w.dphi = angle;
-  w.a1x = std::cos(zero);
-  w.a1y = std::sin(zero);
-  w.a2x = std::cos(zero + angle);
-  w.a2y = std::sin(zero + angle);
-  w.n1x = -std::sin(zero);
-  w.n1y = std::cos(zero);
-  w.n2x = std::sin(zero + angle);
-  w.n2y = -std::cos(zero + angle);
-  return w;
-}
 
 // volumes/UnplacedTube.h:41-60 (calculateCached) and constructor :64-88.
 TubeP MakeTube(double rmin, double rmax, double z, double sphi, double dphi) {
@@ -104,6 +90,21 @@ void TubeExtent(const TubeP &t, double lo[3], double hi[3]) {
 }
 
 }  // namespace
+
+// source/Wedge.cpp:16-43
+WedgeP MakeWedge(double angle, double zero) {
+  WedgeP w;
+  w.dphi = angle;
+  w.a1x = std::cos(zero);
+  w.a1y = std::sin(zero);
+  w.a2x = std::cos(zero + angle);
+  w.a2y = std::sin(zero + angle);
+  w.n1x = -std::sin(zero);
+  w.n1y = std::cos(zero);
+  w.n2x = std::sin(zero + angle);
+  w.n2y = -std::cos(zero + angle);
+  return w;
+}
 
 std::vector<double> BuildParams(int type, const double *c, int n, std::string *err) {
   std::vector<double> none;

@@ -181,6 +181,48 @@ std::vector<double> BuildZooParams(int type, const double *c, int n, std::string
       q.fullTheta = fullTheta ? 1. : 0.;
       return Pack(q);
     }
+    case VG_CONE: {
+      if (n != 7 || !(c[4] > 0 && c[0] >= 0 && c[2] >= 0 && c[1] >= c[0] && c[3] >= c[2] && c[6] > 0))
+        return Fail(err, "cone needs dz > 0, 0 <= rmin <= rmax at both ends, dphi > 0");
+      // volumes/UnplacedCone.h:131-197 (constructor)
+      ConeP q;
+      std::memset(&q, 0, sizeof(q));
+      const double kAngTol = 1e-9;
+      q.rmin1 = c[0];
+      q.rmax1 = c[1];
+      q.rmin2 = c[2];
+      q.rmax2 = c[3];
+      q.dz = c[4];
+      q.sphi = c[5];
+      q.dphi = c[6];
+      q.w = MakeWedge(q.dphi, q.sphi);
+      const double hDPhi = 0.5 * q.dphi;
+      const double cPhi = q.sphi + hDPhi;
+      const double ePhi = q.sphi + q.dphi;
+      q.sinCPhi = std::sin(cPhi);
+      q.cosCPhi = std::cos(cPhi);
+      q.cosHDPhi = std::cos(hDPhi);
+      q.cosHDPhiIT = std::cos(hDPhi - 0.5 * kAngTol);
+      q.cosHDPhiOT = std::cos(hDPhi + 0.5 * kAngTol);
+      q.sinSPhi = std::sin(q.sphi);
+      q.cosSPhi = std::cos(q.sphi);
+      q.sinEPhi = std::sin(ePhi);
+      q.cosEPhi = std::cos(ePhi);
+      q.tanRMin = (q.rmin2 - q.rmin1) * 0.5 / q.dz;
+      q.secRMin = std::sqrt(1.0 + q.tanRMin * q.tanRMin);
+      q.invSecRMin = 1. / q.secRMin;
+      q.tanRMax = (q.rmax2 - q.rmax1) * 0.5 / q.dz;
+      q.secRMax = std::sqrt(1.0 + q.tanRMax * q.tanRMax);
+      q.invSecRMax = 1. / q.secRMax;
+      q.innerSlope = -(q.rmin1 - q.rmin2) / (2. * q.dz);
+      q.outerSlope = -(q.rmax1 - q.rmax2) / (2. * q.dz);
+      q.innerOffset = q.rmin2 - q.innerSlope * q.dz;
+      q.outerOffset = q.rmax2 - q.outerSlope * q.dz;
+      q.fullPhi = q.dphi == kTwoPi ? 1. : 0.;
+      q.phiTreatment = q.dphi < 2. * M_PI ? 1. : 0.;
+      q.rminTreatment = (q.rmin1 > 0 || q.rmin2 > 0) ? 1. : 0.;
+      return Pack(q);
+    }
     case VG_TRAPEZOID: {
       if (n != 11 || !(c[0] > 0 && c[3] > 0 && c[4] > 0 && c[5] > 0 && c[7] > 0 && c[8] > 0 && c[9] > 0))
         return Fail(err, "trapezoid needs 11 parameters with positive lengths");
@@ -236,6 +278,14 @@ bool ZooExtent(int type, const double *b, double lo[3], double hi[3]) {
       const ParabP *q = reinterpret_cast<const ParabP *>(b);
       lo[0] = -q->dx; lo[1] = -q->dy; lo[2] = -q->dz;
       hi[0] = q->dx; hi[1] = q->dy; hi[2] = q->dz;
+      return true;
+    }
+    case VG_CONE: {
+      // UnplacedCone::Extent, volumes/UnplacedCone.h:324-328
+      const ConeP *q = reinterpret_cast<const ConeP *>(b);
+      const double m = q->rmax1 > q->rmax2 ? q->rmax1 : q->rmax2;
+      lo[0] = -m; lo[1] = -m; lo[2] = -q->dz;
+      hi[0] = m; hi[1] = m; hi[2] = q->dz;
       return true;
     }
     case VG_SPHERE: {

@@ -307,6 +307,7 @@ int LaunchQuery(const vg_geometry *g, int32_t placed, int q, const QArgs &a, hip
     case VG_PARABOLOID: return LaunchShape<ParabS>(g, placed, block, pr, q, a, s);
     case VG_TRAPEZOID: return LaunchShape<TrapS>(g, placed, block, pr, q, a, s);
     case VG_SPHERE: return LaunchShape<SphereS>(g, placed, block, pr, q, a, s);
+    case VG_CONE: return LaunchShape<ConeS>(g, placed, block, pr, q, a, s);
   }
   return SetError(VG_ERR_UNSUPPORTED, "no device kernels for shape type " + std::to_string(pr.type));
 }

@@ -286,6 +286,20 @@ class DeviceGeometry:
             pass
 
 
+def single_shape_flat(unplaced_list, transforms, world_half=1e4):
+    """Flat arrays of a world box holding one placement per (shape,
+    transform); returns (FlatArrays, [placed ids], GeoManager)."""
+    gm = GeoManager()
+    world_lv = LogicalVolume("world", UnplacedBox(world_half, world_half, world_half), gm)
+    ids = []
+    for k, (u, xf) in enumerate(zip(unplaced_list, transforms)):
+        lv = LogicalVolume(f"shape{k}", u, gm)
+        pv = world_lv.PlaceDaughter(f"shape{k}", lv, xf)
+        ids.append(pv.id)
+    gm.SetWorld(world_lv.Place("world"))
+    return gm.CloseGeometry(), ids, gm
+
+
 def single_shape_geometry(unplaced_list, transforms, world_half=1e4):
     """World box holding one placement per (shape, transform); returns
     (DeviceGeometry, [placed ids]).  Used by shape benchmarks and tests."""
