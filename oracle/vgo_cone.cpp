@@ -14,6 +14,8 @@
 // IsFullPhi is dphi == 2 pi exactly (UnplacedCone.h:258); the phi treatment
 // of Contains/Inside uses dphi < 2 pi.  stepMax is not used by the
 // reference's cone.
+#include <vector>
+
 #include "vgo.h"
 #include "vgo_wedge.h"
 
@@ -510,7 +512,11 @@ struct Cone : Shape {
     return sphiD;
   }
 
-  double SafetyToIn(V3 p) const override {
+  double SafetyToIn(V3 p) const override { return SafetyToInImpl(p, false); }
+
+  // SafetyToInUSOLIDS<Backend, ForPolycone>: the polycone asks for the
+  // radial and phi parts only (no z term) for the section holding the point.
+  double SafetyToInImpl(V3 p, bool forPolycone) const {
     double safe;
     const double rho = std::sqrt(p.x * p.x + p.y * p.y);
     const double safeZ = Abs(p.z) - dz;
@@ -524,7 +530,7 @@ struct Cone : Shape {
       const double pRMax = tanRMax * p.z + (rmax1 + rmax2) * 0.5;
       safe = (rho - pRMax) * invSecRMax;
     }
-    if (safeZ > safe) safe = safeZ;
+    if (!forPolycone && safeZ > safe) safe = safeZ;
     if (!fullPhi && rho != 0) {
       const double cosPsi = p.x * cosCPhi + p.y * sinCPhi;
       if (cosPsi < cosHDPhi * rho) {
@@ -565,7 +571,195 @@ struct Cone : Shape {
   }
 };
 
+// ---------------------------------------------------------------- Polycone
+// volumes/UnplacedPolycone.h:32-261, source/UnplacedPolycone.cpp:27-180 (Init)
+// and :755-774 (Extent), volumes/kernel/PolyconeImplementation.h:58-494.
+struct Polycone : Shape {
+  std::vector<Cone> sec;
+  std::vector<double> shift;
+  std::vector<double> zs;  // nSections + 1
+
+  // construction parameters: sphi dphi nz rmin[nz] rmax[nz] z[nz]
+  bool Init(const double *a, int n) {
+    if (n < 3) return false;
+    const int nz = (int)a[2];
+    if (nz < 2 || n != 3 + 3 * nz) return false;
+    const double phiStart = a[0], phiTotal = a[1];
+    const double *rmin = a + 3, *rmax = a + 3 + nz, *z = a + 3 + 2 * nz;
+    double prevZ = z[0], prevRmax = 0, prevRmin = 0;
+    const int dirZ = z[1] < z[0] ? -1 : 1;
+    for (int i = 0; i < nz; ++i) {
+      if (i == 0) {
+        zs.push_back(z[i]);
+      } else if (((z[i] > prevZ) && (dirZ > 0)) || ((z[i] < prevZ) && (dirZ < 0))) {
+        const double c[7] = {prevRmin, prevRmax, rmin[i], rmax[i], (z[i] - prevZ) / 2, phiStart, phiTotal};
+        sec.emplace_back(c);
+        zs.push_back(z[i]);
+        const int zi = (int)zs.size() - 1;
+        shift.push_back(zs[zi - 1] + 0.5 * (zs[zi] - zs[zi - 1]));
+      }
+      prevZ = z[i];
+      prevRmin = rmin[i];
+      prevRmax = rmax[i];
+    }
+    return !sec.empty();
+  }
+  int NSec() const { return (int)sec.size(); }
+  // GetSectionIndex, UnplacedPolycone.h:111-120
+  int SectionIndex(double z) const {
+    if (z < zs[0]) return -1;
+    for (int i = 0; i < NSec(); ++i)
+      if (z >= zs[i] && z <= zs[i + 1]) return i;
+    return -2;
+  }
+  static V3 Shift(V3 p, double s) { return V3{p.x - 0., p.y - 0., p.z - s}; }
+
+  bool Contains(V3 p) const override {
+    if (p.z < zs[0] || p.z > zs[NSec()]) return false;
+    const int i = SectionIndex(p.z);
+    return sec[i].Contains(Shift(p, shift[i]));
+  }
+  // The reference's polycone Inside reports only inside or outside.
+  int Inside(V3 p) const override { return Contains(p) ? kInside : kOutside; }
+
+  double DistanceToIn(V3 p, V3 v, double stepMax) const override {
+    double distance = kInfinity;
+    int increment = (v.z > 0) ? 1 : -1;
+    if (std::fabs(v.z) < kTolerance) increment = 0;
+    int index = SectionIndex(p.z);
+    if (index == -1) index = 0;
+    if (index == -2) index = NSec() - 1;
+    do {
+      distance = sec[index].DistanceToIn(Shift(p, shift[index]), v, stepMax);
+      if (distance < kInfinity || !increment) break;
+      index += increment;
+    } while (index >= 0 && index < NSec());
+    return distance;
+  }
+
+  double DistanceToOut(V3 point, V3 dir, double stepMax) const override {
+    if (NSec() == 1) return sec[0].DistanceToOut(Shift(point, shift[0]), dir, stepMax);
+    const int indexLow = SectionIndex(point.z - kTolerance);
+    const int indexHigh = SectionIndex(point.z + kTolerance);
+    int index;
+    if (indexLow != indexHigh && indexLow >= 0) {
+      index = sec[indexLow].Contains(Shift(point, shift[indexLow])) ? indexLow : indexHigh;
+    } else {
+      index = indexLow;
+      if (index < 0) index = SectionIndex(point.z);
+    }
+    if (index < 0) return 0.;
+    double totalDistance = 0.;
+    int increment = (dir.z > 0) ? 1 : -1;
+    if (std::fabs(dir.z) < kTolerance) increment = 0;
+    int istep = 0;
+    V3 pn = point;
+    do {
+      if ((totalDistance != 0) || (istep < 2)) {
+        pn = V3{point.x + totalDistance * dir.x, point.y + totalDistance * dir.y, point.z + totalDistance * dir.z};
+        pn.z -= shift[index];
+        if (sec[index].Inside(pn) == kOutside) break;
+      } else {
+        pn.z -= shift[index];
+      }
+      istep++;
+      const double dist = sec[index].DistanceToOut(pn, dir, stepMax);
+      if (std::fabs(dist) < 0.5 * kTolerance) {
+        int index1 = index;
+        if ((index > 0) && (index < NSec() - 1)) {
+          index1 += increment;
+        } else {
+          if ((index == 0) && (increment > 0)) index1 += increment;
+          if ((index == NSec() - 1) && (increment < 0)) index1 += increment;
+        }
+        const double s = totalDistance + dist;
+        V3 pte{point.x + s * dir.x, point.y + s * dir.y, point.z + s * dir.z};
+        pte.z -= shift[index1];
+        if (!sec[index1].Contains(pte)) break;
+      }
+      totalDistance += dist;
+      index += increment;
+    } while (increment != 0 && index >= 0 && index < NSec());
+    return totalDistance;
+  }
+
+  double SafetyToIn(V3 p) const override {
+    int index = SectionIndex(p.z);
+    bool needZ = false;
+    if (index < 0) {
+      needZ = true;
+      if (index == -1) index = 0;
+      if (index == -2) index = NSec() - 1;
+    }
+    if (needZ) return sec[index].SafetyToInImpl(Shift(p, shift[index]), false);
+    double safety = sec[index].SafetyToInImpl(Shift(p, shift[index]), true);
+    if (safety < kTolerance) return safety;
+    double minSafety = safety;
+    double zbase = zs[index + 1];
+    for (int i = index + 1; i < NSec(); ++i) {
+      const double dz = zs[i] - zbase;
+      if (dz >= minSafety) break;
+      safety = sec[i].SafetyToInImpl(Shift(p, shift[i]), false);
+      if (safety < minSafety) minSafety = safety;
+    }
+    if (index > 0) {
+      zbase = zs[index - 1];
+      for (int i = index - 1; i >= 0; --i) {
+        const double dz = zbase - zs[i];
+        if (dz >= minSafety) break;
+        safety = sec[i].SafetyToInImpl(Shift(p, shift[i]), false);
+        if (safety < minSafety) minSafety = safety;
+      }
+    }
+    return minSafety;
+  }
+
+  double SafetyToOut(V3 point) const override {
+    const int index = SectionIndex(point.z);
+    if (index < 0) return 0;
+    double minSafety = sec[index].SafetyToOut(Shift(point, shift[index]));
+    if (minSafety == kInfinity) return 0.;
+    if (minSafety < kTolerance) return 0.;
+    double zbase = zs[index + 1];
+    for (int i = index + 1; i < NSec(); ++i) {
+      const double dz = zs[i] - zbase;
+      if (dz >= minSafety) break;
+      const double safety = sec[i].SafetyToInImpl(Shift(point, shift[i]), false);
+      if (safety < minSafety) minSafety = safety;
+    }
+    if (index > 0) {
+      zbase = zs[index - 1];
+      for (int i = index - 1; i >= 0; --i) {
+        const double dz = zbase - zs[i];
+        if (dz >= minSafety) break;
+        const double safety = sec[i].SafetyToInImpl(Shift(point, shift[i]), false);
+        if (safety < minSafety) minSafety = safety;
+      }
+    }
+    return minSafety;
+  }
+
+  void Extent(V3 &lo, V3 &hi) const override {
+    double maxR = 0;
+    for (const Cone &c : sec) {
+      if (maxR < c.rmax1) maxR = c.rmax1;
+      if (maxR < c.rmax2) maxR = c.rmax2;
+    }
+    lo = V3{-maxR, -maxR, zs[0]};
+    hi = V3{maxR, maxR, zs[NSec()]};
+  }
+};
+
 }  // namespace
+
+Shape *MakePolycone(const double *p, int n) {
+  auto *s = new Polycone;
+  if (!s->Init(p, n)) {
+    delete s;
+    return nullptr;
+  }
+  return s;
+}
 
 Shape *MakeCone(const double *p, int n) {
   if (n != 7) return nullptr;

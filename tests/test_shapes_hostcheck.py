@@ -15,7 +15,7 @@ from vecgeom_amd import geometry as G
 from vecgeom_amd import workloads as W
 
 LIB = os.path.join(ROOT, "tests", "native", "libnavhost.so")
-DEVICE_TYPES = {1, 2, 3, 4, 5, 6, 7, 8, 9}
+DEVICE_TYPES = {1, 2, 3, 4, 5, 6, 7, 8, 9, 11}
 QUERIES = [("contains", po.Q_CONTAINS, 0), ("inside", po.Q_INSIDE, 1), ("dist_in", po.Q_DIST_IN, 2),
            ("dist_out", po.Q_DIST_OUT, 3), ("safe_in", po.Q_SAFE_IN, 4), ("safe_out", po.Q_SAFE_OUT, 5)]
 EXTRA = [  # phi-sectioned and degenerate variants beyond the zoo
@@ -24,6 +24,10 @@ EXTRA = [  # phi-sectioned and degenerate variants beyond the zoo
     ("ConeHalf", G.UnplacedCone(5, 20, 10, 40, 30, 0.5, np.pi)),
     ("ConeApex", G.UnplacedCone(0, 30, 0, 1e-4, 40, 0, 2 * np.pi)),
     ("TubePhi", G.UnplacedTube(10, 50, 50, 0.4, 2.0)),
+    ("PolyconePhi", G.UnplacedPolycone(0.5, 4.0, rmin=[5, 5, 10, 10, 8], rmax=[20, 30, 30, 40, 40],
+                                       z=[-50, -40, -30, -20, 0])),
+    ("PolyconeDupZ", G.UnplacedPolycone(0.0, 2 * np.pi, rmin=[5, 5, 8, 8], rmax=[20, 25, 25, 30], z=[-30, 0, 0, 30])),
+    ("PolyconeOne", G.UnplacedPolycone(0.0, 2 * np.pi, rmin=[0, 4], rmax=[20, 30], z=[-30, 30])),
 ]
 
 

@@ -5,3 +5,4 @@
 #include "shapes_trap.hpp"
 #include "shapes_sphere.hpp"
 #include "shapes_cone.hpp"
+#include "shapes_polycone.hpp"

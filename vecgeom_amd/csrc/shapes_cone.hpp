@@ -449,7 +449,12 @@ struct ConeS {
     return snxt;
   }
 
-  static VGA_HD double SafetyToIn(const P &c, V3 p) {
+  static VGA_HD double SafetyToIn(const P &c, V3 p) { return SafetyToInImpl(c, p, false); }
+
+  // SafetyToInUSOLIDS<Backend, ForPolycone> (ConeImplementation.h:1908-1996):
+  // a polycone asks for the radial and phi parts only of the section that
+  // holds the point.
+  static VGA_HD double SafetyToInImpl(const P &c, V3 p, bool forPolycone) {
     double safe;
     const double rho = Sqrt(p.x * p.x + p.y * p.y);
     const double safeZ = Abs(p.z) - c.dz;
@@ -463,7 +468,7 @@ struct ConeS {
       const double pRMax = c.tanRMax * p.z + (c.rmax1 + c.rmax2) * 0.5;
       safe = (rho - pRMax) * c.invSecRMax;
     }
-    if (safeZ > safe) safe = safeZ;
+    if (!forPolycone && safeZ > safe) safe = safeZ;
     if (c.fullPhi == 0 && rho != 0) {
       const double cosPsi = p.x * c.cosCPhi + p.y * c.sinCPhi;
       if (cosPsi < c.cosHDPhi * rho) {

@@ -94,9 +94,13 @@ struct NavDev {
       using S = ConeS;                         \
       BODY;                                    \
     }                                          \
+    case VG_POLYCONE: {                        \
+      using S = PolyconeS;                     \
+      BODY;                                    \
+    }                                          \
   }
 
-#define VGA_P(blk) (*reinterpret_cast<const typename S::P *>(blk))
+#define VGA_P(blk) (ParamView<S>::Get(blk))
 
 VGA_NAV bool DContains(int type, const double *blk, V3 p) {
   VGA_SHAPE_CASES(type, return S::Contains(VGA_P(blk), p));

@@ -65,6 +65,48 @@ struct TrapBuild {
 
 }  // namespace
 
+// UnplacedCone constructor (volumes/UnplacedCone.h:131-197): the cached
+// values of a cone with parameters c = rmin1 rmax1 rmin2 rmax2 dz sphi dphi.
+ConeP MakeConeP(const double *c) {
+  ConeP q;
+  std::memset(&q, 0, sizeof(q));
+  const double kAngTol = 1e-9;
+  q.rmin1 = c[0];
+  q.rmax1 = c[1];
+  q.rmin2 = c[2];
+  q.rmax2 = c[3];
+  q.dz = c[4];
+  q.sphi = c[5];
+  q.dphi = c[6];
+  q.w = MakeWedge(q.dphi, q.sphi);
+  const double hDPhi = 0.5 * q.dphi;
+  const double cPhi = q.sphi + hDPhi;
+  const double ePhi = q.sphi + q.dphi;
+  q.sinCPhi = std::sin(cPhi);
+  q.cosCPhi = std::cos(cPhi);
+  q.cosHDPhi = std::cos(hDPhi);
+  q.cosHDPhiIT = std::cos(hDPhi - 0.5 * kAngTol);
+  q.cosHDPhiOT = std::cos(hDPhi + 0.5 * kAngTol);
+  q.sinSPhi = std::sin(q.sphi);
+  q.cosSPhi = std::cos(q.sphi);
+  q.sinEPhi = std::sin(ePhi);
+  q.cosEPhi = std::cos(ePhi);
+  q.tanRMin = (q.rmin2 - q.rmin1) * 0.5 / q.dz;
+  q.secRMin = std::sqrt(1.0 + q.tanRMin * q.tanRMin);
+  q.invSecRMin = 1. / q.secRMin;
+  q.tanRMax = (q.rmax2 - q.rmax1) * 0.5 / q.dz;
+  q.secRMax = std::sqrt(1.0 + q.tanRMax * q.tanRMax);
+  q.invSecRMax = 1. / q.secRMax;
+  q.innerSlope = -(q.rmin1 - q.rmin2) / (2. * q.dz);
+  q.outerSlope = -(q.rmax1 - q.rmax2) / (2. * q.dz);
+  q.innerOffset = q.rmin2 - q.innerSlope * q.dz;
+  q.outerOffset = q.rmax2 - q.outerSlope * q.dz;
+  q.fullPhi = q.dphi == kTwoPi ? 1. : 0.;
+  q.phiTreatment = q.dphi < 2. * M_PI ? 1. : 0.;
+  q.rminTreatment = (q.rmin1 > 0 || q.rmin2 > 0) ? 1. : 0.;
+  return q;
+}
+
 std::vector<double> BuildZooParams(int type, const double *c, int n, std::string *err) {
   switch (type) {
     case VG_ORB: {
@@ -184,44 +226,48 @@ std::vector<double> BuildZooParams(int type, const double *c, int n, std::string
     case VG_CONE: {
       if (n != 7 || !(c[4] > 0 && c[0] >= 0 && c[2] >= 0 && c[1] >= c[0] && c[3] >= c[2] && c[6] > 0))
         return Fail(err, "cone needs dz > 0, 0 <= rmin <= rmax at both ends, dphi > 0");
-      // volumes/UnplacedCone.h:131-197 (constructor)
-      ConeP q;
-      std::memset(&q, 0, sizeof(q));
-      const double kAngTol = 1e-9;
-      q.rmin1 = c[0];
-      q.rmax1 = c[1];
-      q.rmin2 = c[2];
-      q.rmax2 = c[3];
-      q.dz = c[4];
-      q.sphi = c[5];
-      q.dphi = c[6];
-      q.w = MakeWedge(q.dphi, q.sphi);
-      const double hDPhi = 0.5 * q.dphi;
-      const double cPhi = q.sphi + hDPhi;
-      const double ePhi = q.sphi + q.dphi;
-      q.sinCPhi = std::sin(cPhi);
-      q.cosCPhi = std::cos(cPhi);
-      q.cosHDPhi = std::cos(hDPhi);
-      q.cosHDPhiIT = std::cos(hDPhi - 0.5 * kAngTol);
-      q.cosHDPhiOT = std::cos(hDPhi + 0.5 * kAngTol);
-      q.sinSPhi = std::sin(q.sphi);
-      q.cosSPhi = std::cos(q.sphi);
-      q.sinEPhi = std::sin(ePhi);
-      q.cosEPhi = std::cos(ePhi);
-      q.tanRMin = (q.rmin2 - q.rmin1) * 0.5 / q.dz;
-      q.secRMin = std::sqrt(1.0 + q.tanRMin * q.tanRMin);
-      q.invSecRMin = 1. / q.secRMin;
-      q.tanRMax = (q.rmax2 - q.rmax1) * 0.5 / q.dz;
-      q.secRMax = std::sqrt(1.0 + q.tanRMax * q.tanRMax);
-      q.invSecRMax = 1. / q.secRMax;
-      q.innerSlope = -(q.rmin1 - q.rmin2) / (2. * q.dz);
-      q.outerSlope = -(q.rmax1 - q.rmax2) / (2. * q.dz);
-      q.innerOffset = q.rmin2 - q.innerSlope * q.dz;
-      q.outerOffset = q.rmax2 - q.outerSlope * q.dz;
-      q.fullPhi = q.dphi == kTwoPi ? 1. : 0.;
-      q.phiTreatment = q.dphi < 2. * M_PI ? 1. : 0.;
-      q.rminTreatment = (q.rmin1 > 0 || q.rmin2 > 0) ? 1. : 0.;
-      return Pack(q);
+      return Pack(MakeConeP(c));
+    }
+    case VG_POLYCONE: {
+      // UnplacedPolycone::Init, source/UnplacedPolycone.cpp:27-180: one cone
+      // section per rise in z, shifted to the middle of its z range.
+      if (n < 3) return Fail(err, "polycone needs sphi dphi nz rmin[nz] rmax[nz] z[nz]");
+      const int nz = (int)c[2];
+      if (nz < 2 || n != 3 + 3 * nz) return Fail(err, "polycone needs nz >= 2 and 3 + 3 nz parameters");
+      const double *rmin = c + 3, *rmax = c + 3 + nz, *z = c + 3 + 2 * nz;
+      for (int i = 0; i < nz; ++i)
+        if (!(rmin[i] >= 0 && rmax[i] >= rmin[i])) return Fail(err, "polycone needs 0 <= rmin <= rmax per plane");
+      if (!(c[1] > 0)) return Fail(err, "polycone needs dphi > 0");
+      std::vector<double> zs;
+      std::vector<ConeP> cones;
+      std::vector<double> shifts;
+      double prevZ = z[0], prevRmax = 0, prevRmin = 0;
+      const int dirZ = z[1] < z[0] ? -1 : 1;
+      for (int i = 0; i < nz; ++i) {
+        if (i == 0) {
+          zs.push_back(z[i]);
+        } else if (((z[i] > prevZ) && (dirZ > 0)) || ((z[i] < prevZ) && (dirZ < 0))) {
+          const double cc[7] = {prevRmin, prevRmax, rmin[i], rmax[i], (z[i] - prevZ) / 2, c[0], c[1]};
+          cones.push_back(MakeConeP(cc));
+          zs.push_back(z[i]);
+          const size_t zi = zs.size() - 1;
+          shifts.push_back(zs[zi - 1] + 0.5 * (zs[zi] - zs[zi - 1]));
+        }
+        prevZ = z[i];
+        prevRmin = rmin[i];
+        prevRmax = rmax[i];
+      }
+      const int ns = (int)cones.size();
+      if (ns == 0) return Fail(err, "polycone has no section");
+      std::vector<double> blk(PolyconeS::SecBase(ns) + PolyconeS::kSecDoubles * (size_t)ns, 0.);
+      blk[0] = ns;
+      for (int i = 0; i <= ns; ++i) blk[2 + i] = zs[i];
+      for (int i = 0; i < ns; ++i) {
+        double *r = blk.data() + PolyconeS::SecBase(ns) + PolyconeS::kSecDoubles * (size_t)i;
+        std::memcpy(r, &cones[i], sizeof(ConeP));
+        r[40] = shifts[i];
+      }
+      return blk;
     }
     case VG_TRAPEZOID: {
       if (n != 11 || !(c[0] > 0 && c[3] > 0 && c[4] > 0 && c[5] > 0 && c[7] > 0 && c[8] > 0 && c[9] > 0))
@@ -286,6 +332,20 @@ bool ZooExtent(int type, const double *b, double lo[3], double hi[3]) {
       const double m = q->rmax1 > q->rmax2 ? q->rmax1 : q->rmax2;
       lo[0] = -m; lo[1] = -m; lo[2] = -q->dz;
       hi[0] = m; hi[1] = m; hi[2] = q->dz;
+      return true;
+    }
+    case VG_POLYCONE: {
+      // UnplacedPolycone::Extent, source/UnplacedPolycone.cpp:755-774
+      const PolyconeP q{b};
+      const int ns = PolyconeS::NSec(q);
+      double maxR = 0;
+      for (int i = 0; i < ns; ++i) {
+        const ConeP &c = PolyconeS::Sec(q, i);
+        if (maxR < c.rmax1) maxR = c.rmax1;
+        if (maxR < c.rmax2) maxR = c.rmax2;
+      }
+      lo[0] = -maxR; lo[1] = -maxR; lo[2] = PolyconeS::Z(q, 0);
+      hi[0] = maxR; hi[1] = maxR; hi[2] = PolyconeS::Z(q, ns);
       return true;
     }
     case VG_SPHERE: {

@@ -189,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void PlainQueryKernel(const PlacedRec *__re
   const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= a.n) return;
   const PlacedRec &pr = placed[pid];
-  const typename S::P &P = *reinterpret_cast<const typename S::P *>(params + pr.param_off);
+  const auto &P = ParamView<S>::Get(params + pr.param_off);
   const V3 p{a.x[i], a.y[i], a.z[i]};
   V3 d{0, 0, 0};
   if constexpr (NeedsDir(Q)) d = V3{a.dx[i], a.dy[i], a.dz[i]};
@@ -271,7 +271,11 @@ int LaunchShape(const vg_geometry *g, int32_t pid, const double *block, const Pl
     return LaunchShapePlain<S>(g, pid, q, a, s);
   }
   typename S::P P;
-  std::memcpy(&P, block, sizeof(P));
+  if constexpr (ParamView<S>::kIndirect) {
+    P = ParamView<S>::Get(g->d_params + pr.param_off);
+  } else {
+    std::memcpy(&P, block, sizeof(P));
+  }
   switch (q) {
     case kQContains: return LaunchXf<S, kQContains>(P, pr, a, s);
     case kQInside: return LaunchXf<S, kQInside>(P, pr, a, s);
@@ -308,6 +312,7 @@ int LaunchQuery(const vg_geometry *g, int32_t placed, int q, const QArgs &a, hip
     case VG_TRAPEZOID: return LaunchShape<TrapS>(g, placed, block, pr, q, a, s);
     case VG_SPHERE: return LaunchShape<SphereS>(g, placed, block, pr, q, a, s);
     case VG_CONE: return LaunchShape<ConeS>(g, placed, block, pr, q, a, s);
+    case VG_POLYCONE: return LaunchShape<PolyconeS>(g, placed, block, pr, q, a, s);
   }
   return SetError(VG_ERR_UNSUPPORTED, "no device kernels for shape type " + std::to_string(pr.type));
 }
