@@ -571,6 +571,6 @@ struct Trapezoid : Shape {
 };
 
 Shape *MakeTrapezoid(const double *p, int n) { return n == 11 ? new Trapezoid(p) : nullptr; }
-Shape *MakeTorus(const double *, int) { return nullptr; }
+
 Shape *MakePolyhedron(const double *, int) { return nullptr; }
 }  // namespace vgo

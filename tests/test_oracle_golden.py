@@ -77,3 +77,36 @@ def test_oracle_matches_reference_random(oracle, reference, case):
         b = run_query(r, qn, gg)
         ok = same(a, b)
         assert ok.all(), f"{case} {qn}: {int((~ok).sum())} of {n} lanes differ"
+
+
+def to_local(xf12, p, direction=False):
+    """The general placement transform in the reference's operation order
+    (base/Transformation3D.h:469-478, 537-562): local = R^T (p - t)."""
+    t, r = xf12[:3], xf12[3:]
+    m = p if direction else p - t[:, None]
+    lx = m[0] * r[0]
+    ly = m[0] * r[1]
+    lz = m[0] * r[2]
+    lx = lx + m[1] * r[3]
+    ly = ly + m[1] * r[4]
+    lz = lz + m[1] * r[5]
+    lx = lx + m[2] * r[6]
+    ly = ly + m[2] * r[7]
+    lz = lz + m[2] * r[8]
+    return np.stack([lx, ly, lz])
+
+
+def test_torus_rotated_distance_to_in_at_local_level(oracle, reference):
+    """The reference's placed Torus DistanceToIn applies the point transform
+    to the direction (volumes/kernel/TorusImplementation.h:906).  The oracle
+    rotates the direction; its placed result must equal the reference's
+    unplaced DistanceToIn on the locally transformed point and direction,
+    where the reference's defect is inert."""
+    g = load_golden("shape_Torus_rot.npz")
+    xf = np.asarray(g["xf"], dtype=np.float64)
+    o = oracle.placed(int(g["type"]), g["params"], xf)
+    r = reference.placed(int(g["type"]), g["params"], np.array([0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1.0]))
+    lp, ld = to_local(xf, g["p"]), to_local(xf, g["d"], direction=True)
+    want = r.query(po.Q_DIST_IN, lp, ld, g["sm"])
+    got = o.query(po.Q_DIST_IN, g["p"], g["d"], g["sm"])
+    assert same(got, want).all()

@@ -15,7 +15,7 @@ from vecgeom_amd import geometry as G
 from vecgeom_amd import workloads as W
 
 LIB = os.path.join(ROOT, "tests", "native", "libnavhost.so")
-DEVICE_TYPES = {1, 2, 3, 4, 5, 6, 7, 8, 9, 11}
+DEVICE_TYPES = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11}
 QUERIES = [("contains", po.Q_CONTAINS, 0), ("inside", po.Q_INSIDE, 1), ("dist_in", po.Q_DIST_IN, 2),
            ("dist_out", po.Q_DIST_OUT, 3), ("safe_in", po.Q_SAFE_IN, 4), ("safe_out", po.Q_SAFE_OUT, 5)]
 EXTRA = [  # phi-sectioned and degenerate variants beyond the zoo
@@ -27,8 +27,25 @@ EXTRA = [  # phi-sectioned and degenerate variants beyond the zoo
     ("PolyconePhi", G.UnplacedPolycone(0.5, 4.0, rmin=[5, 5, 10, 10, 8], rmax=[20, 30, 30, 40, 40],
                                        z=[-50, -40, -30, -20, 0])),
     ("PolyconeDupZ", G.UnplacedPolycone(0.0, 2 * np.pi, rmin=[5, 5, 8, 8], rmax=[20, 25, 25, 30], z=[-30, 0, 0, 30])),
+    ("TorusPhi", G.UnplacedTorus(5, 10, 50, 0.3, 4.0)),
+    ("TorusSolid", G.UnplacedTorus(0, 10, 40, 0, 2 * np.pi)),
     ("PolyconeOne", G.UnplacedPolycone(0.0, 2 * np.pi, rmin=[0, 4], rmax=[20, 30], z=[-30, 30])),
 ]
+
+
+# Distances of the Torus come from a quartic solved with cbrt/atan2/sin/cos;
+# library builds of those differ in the last ulp (host compilers differ from
+# each other already), so they are held to the FP64 distance tolerance
+# max(1e-9 mm, 1e-12 relative) instead of bit equality.
+TOLERANT = {("Torus", "dist_in"), ("Torus", "dist_out")}
+
+
+def close(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    both = (a == b) | (np.isnan(a) & np.isnan(b))
+    with np.errstate(invalid="ignore"):
+        return both | (np.abs(a - b) <= np.maximum(1e-9, 1e-12 * np.abs(b)))
 
 
 class HostLib:
@@ -59,7 +76,7 @@ def run_host(host, nav, pid, q, p, d=None, sm=None):
     return out
 
 
-def compare(oracle, host, u, xf, inputs):
+def compare(oracle, host, u, xf, inputs, family=""):
     flat, ids, _ = G.single_shape_flat([u], [xf])
     nav = po.Nav(host, flat)
     o = oracle.placed(u.type, u.params, xf.array())
@@ -71,7 +88,7 @@ def compare(oracle, host, u, xf, inputs):
             args = (inputs["p"], inputs["d"] if name == "dist_in" else None, inputs["sm"] if name == "dist_in" else None)
         want = o.query(q, *[a for a in args if a is not None]) if args[1] is not None else o.query(q, args[0])
         got = run_host(host, nav, ids[0], hq, *args)
-        ok = same(got, want)
+        ok = close(got, want) if (family, name) in TOLERANT else same(got, want)
         assert ok.all(), f"{name}: {int((~ok).sum())} lanes differ, first {np.flatnonzero(~ok)[:4]}"
 
 
@@ -81,7 +98,7 @@ def test_host_shapes_on_golden(oracle, host, case):
     if int(g["type"]) not in DEVICE_TYPES:
         pytest.skip("no device kernels for this shape")
     u = G.UnplacedVolume(int(g["type"]), g["params"])
-    compare(oracle, host, u, G.Transformation3D.from_array(g["xf"]), g)
+    compare(oracle, host, u, G.Transformation3D.from_array(g["xf"]), g, case.split("_")[1].replace("C2", ""))
 
 
 @pytest.mark.parametrize("name", [n for n, u in W.zoo_shapes() if u.type in DEVICE_TYPES] + [n for n, _ in EXTRA])
@@ -92,4 +109,4 @@ def test_host_shapes_random(oracle, host, name):
         lo, hi = oracle.placed(u.type, u.params, np.array([0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1.0])).extent()
         inp = W.shape_inputs((lo, hi), xf, 30000, seed=77)
         inp["sm"] = np.full(30000, 1e30)
-        compare(oracle, host, u, xf, inp)
+        compare(oracle, host, u, xf, inp, "Torus" if u.type == 10 else name)

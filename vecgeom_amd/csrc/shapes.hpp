@@ -6,3 +6,4 @@
 #include "shapes_sphere.hpp"
 #include "shapes_cone.hpp"
 #include "shapes_polycone.hpp"
+#include "shapes_torus.hpp"

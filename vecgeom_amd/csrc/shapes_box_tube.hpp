@@ -175,6 +175,9 @@ struct TubeP {
   WedgeP w;
 };
 
+// Host-side construction of a tube's parameter block (vg_params.cpp).
+TubeP MakeTube(double rmin, double rmax, double z, double sphi, double dphi);
+
 struct TubeS {
   using P = TubeP;
   static VGA_HD bool InSector(const P &t, double x, double y) {

@@ -98,6 +98,10 @@ struct NavDev {
       using S = PolyconeS;                     \
       BODY;                                    \
     }                                          \
+    case VG_TORUS: {                           \
+      using S = TorusS;                        \
+      BODY;                                    \
+    }                                          \
   }
 
 #define VGA_P(blk) (ParamView<S>::Get(blk))

@@ -26,35 +26,6 @@ bool Fail(std::string *err, const std::string &m) {
 }
 
 
-// volumes/UnplacedTube.h:41-60 (calculateCached) and constructor :64-88.
-TubeP MakeTube(double rmin, double rmax, double z, double sphi, double dphi) {
-  TubeP t;
-  t.rmin = rmin;
-  t.rmax = rmax;
-  t.z = z;
-  t.sphi = sphi;
-  t.dphi = dphi;
-  t.tolIz = z - kTolerance;
-  t.tolOz = z + kTolerance;
-  t.rmin2 = rmin * rmin;
-  t.rmax2 = rmax * rmax;
-  const double tolOrmin = rmin - kTolerance, tolIrmin = rmin + kTolerance;
-  t.tolOrmin2 = tolOrmin * tolOrmin;
-  t.tolIrmin2 = tolIrmin * tolIrmin;
-  const double tolOrmax = rmax + kTolerance, tolIrmax = rmax - kTolerance;
-  t.tolOrmax2 = tolOrmax * tolOrmax;
-  t.tolIrmax2 = tolIrmax * tolIrmax;
-  t.a1x = std::cos(sphi);
-  t.a1y = std::sin(sphi);
-  t.a2x = std::cos(sphi + dphi);
-  t.a2y = std::sin(sphi + dphi);
-  t.w = MakeWedge(dphi, sphi);
-  // shapetypes/TubeTypes.h:154-160, 196-201 and TubeImplementation.h:71-74
-  t.hasRmin = rmin > 0 ? 1. : 0.;
-  t.hasPhi = dphi < 2 * M_PI ? 1. : 0.;
-  t.smallerThanPi = dphi <= M_PI ? 1. : 0.;
-  return t;
-}
 
 // UnplacedTube::Extent, source/UnplacedTube.cpp:244-296.
 void TubeExtent(const TubeP &t, double lo[3], double hi[3]) {
@@ -90,6 +61,36 @@ void TubeExtent(const TubeP &t, double lo[3], double hi[3]) {
 }
 
 }  // namespace
+
+// volumes/UnplacedTube.h:41-60 (calculateCached) and constructor :64-88.
+TubeP MakeTube(double rmin, double rmax, double z, double sphi, double dphi) {
+  TubeP t;
+  t.rmin = rmin;
+  t.rmax = rmax;
+  t.z = z;
+  t.sphi = sphi;
+  t.dphi = dphi;
+  t.tolIz = z - kTolerance;
+  t.tolOz = z + kTolerance;
+  t.rmin2 = rmin * rmin;
+  t.rmax2 = rmax * rmax;
+  const double tolOrmin = rmin - kTolerance, tolIrmin = rmin + kTolerance;
+  t.tolOrmin2 = tolOrmin * tolOrmin;
+  t.tolIrmin2 = tolIrmin * tolIrmin;
+  const double tolOrmax = rmax + kTolerance, tolIrmax = rmax - kTolerance;
+  t.tolOrmax2 = tolOrmax * tolOrmax;
+  t.tolIrmax2 = tolIrmax * tolIrmax;
+  t.a1x = std::cos(sphi);
+  t.a1y = std::sin(sphi);
+  t.a2x = std::cos(sphi + dphi);
+  t.a2y = std::sin(sphi + dphi);
+  t.w = MakeWedge(dphi, sphi);
+  // shapetypes/TubeTypes.h:154-160, 196-201 and TubeImplementation.h:71-74
+  t.hasRmin = rmin > 0 ? 1. : 0.;
+  t.hasPhi = dphi < 2 * M_PI ? 1. : 0.;
+  t.smallerThanPi = dphi <= M_PI ? 1. : 0.;
+  return t;
+}
 
 // source/Wedge.cpp:16-43
 WedgeP MakeWedge(double angle, double zero) {

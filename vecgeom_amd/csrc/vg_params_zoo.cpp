@@ -228,6 +228,27 @@ std::vector<double> BuildZooParams(int type, const double *c, int n, std::string
         return Fail(err, "cone needs dz > 0, 0 <= rmin <= rmax at both ends, dphi > 0");
       return Pack(MakeConeP(c));
     }
+    case VG_TORUS: {
+      // volumes/UnplacedTorus.h:43-69; a ring torus (rtor > rmax) keeps the
+      // bounding tube's inner radius positive.
+      if (n != 5 || !(c[0] >= 0 && c[1] > c[0] && c[2] > c[1] + kTolerance && c[4] > 0))
+        return Fail(err, "torus needs 0 <= rmin < rmax < rtor and dphi > 0");
+      TorusP q;
+      std::memset(&q, 0, sizeof(q));
+      q.rmin = c[0];
+      q.rmax = c[1];
+      q.rtor = c[2];
+      q.sphi = c[3];
+      q.dphi = c[4];
+      q.rmin2 = q.rmin * q.rmin;
+      q.rmax2 = q.rmax * q.rmax;
+      q.rtor2 = q.rtor * q.rtor;
+      q.w = MakeWedge(q.dphi, q.sphi);
+      // The reference's DistanceToIn uses the HollowTube specialisation of the
+      // bounding tube: rmin treated, phi never.
+      q.btube = MakeTube(q.rtor - q.rmax - kTolerance, q.rtor + q.rmax + kTolerance, q.rmax, 0., kTwoPi);
+      return Pack(q);
+    }
     case VG_POLYCONE: {
       // UnplacedPolycone::Init, source/UnplacedPolycone.cpp:27-180: one cone
       // section per rise in z, shifted to the middle of its z range.
@@ -333,6 +354,13 @@ bool ZooExtent(int type, const double *b, double lo[3], double hi[3]) {
       lo[0] = -m; lo[1] = -m; lo[2] = -q->dz;
       hi[0] = m; hi[1] = m; hi[2] = q->dz;
       return true;
+    }
+    case VG_TORUS: {
+      // UnplacedTorus::Extent = the bounding tube's (with its phi section)
+      const TorusP *q = reinterpret_cast<const TorusP *>(b);
+      const TubeP t = MakeTube(q->rtor - q->rmax - kTolerance, q->rtor + q->rmax + kTolerance, q->rmax, q->sphi,
+                               q->dphi);
+      return ShapeExtent(VG_TUBE, reinterpret_cast<const double *>(&t), lo, hi);
     }
     case VG_POLYCONE: {
       // UnplacedPolycone::Extent, source/UnplacedPolycone.cpp:755-774
