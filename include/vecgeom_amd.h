@@ -217,6 +217,63 @@ int vg_safety_to_out(const vg_geometry *g, int32_t placed, const double *x, cons
 int vg_safety_to_out_minimize(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z,
                               size_t n, double *inout, void *stream);
 
+/* ------------------------------------------- tasks, batches, FP32 storage */
+/*
+ * A task is one basket query on one placed volume.  Query codes: */
+#define VG_Q_CONTAINS 0               /* out: uint8  (VPlacedVolume::Contains)      */
+#define VG_Q_INSIDE 1                 /* out: int32  (Inside)                        */
+#define VG_Q_DISTANCE_TO_IN 2         /* out: double (DistanceToIn, mother frame)    */
+#define VG_Q_DISTANCE_TO_OUT 3        /* out: double (DistanceToOut, local frame)    */
+#define VG_Q_PLACED_DISTANCE_TO_OUT 4 /* out: double (PlacedDistanceToOut)           */
+#define VG_Q_SAFETY_TO_IN 5           /* out: double (SafetyToIn, mother frame)      */
+#define VG_Q_SAFETY_TO_OUT 6          /* out: double (SafetyToOut, local frame)      */
+/* Input element types: VG_F64 = double arrays (the SOA3D<double> of the
+ * reference); VG_F32 = float arrays (FP32 storage: every value is promoted to
+ * double before any arithmetic, so results equal the FP64 form on the same
+ * float-valued inputs).  Outputs are as for the FP64 entry points. */
+#define VG_F64 0
+#define VG_F32 1
+
+typedef struct vg_query_task {
+  int32_t placed;     /* placed-volume id */
+  int32_t query;      /* VG_Q_* */
+  int32_t input_type; /* VG_F64 or VG_F32 */
+  int32_t reserved;
+  uint64_t n;
+  const void *x, *y, *z;    /* device arrays of input_type, length n */
+  const void *dx, *dy, *dz; /* directions (distance queries only) */
+  const double *step_max;   /* distance queries; NULL = kInfinity */
+  void *out;                /* device array, see the query codes */
+} vg_query_task;
+
+/* One task on `stream`: the SOA3D basket virtuals of VPlacedVolume
+ * (volumes/PlacedVolume.h:147-246) with a choice of input storage.  No
+ * counterpart in the reference for VG_F32 inputs. */
+int vg_query(const vg_geometry *g, const vg_query_task *task, void *stream);
+
+/*
+ * A batch: a fixed table of tasks (any mix of placed volumes, queries, basket
+ * sizes and input types) run by ONE kernel launch, each workgroup tile
+ * belonging to one task.  This replaces the per-shape, per-query launches of
+ * Benchmarker::Run*Cuda (source/benchmarking/Benchmarker.cu:86-367).  The
+ * arrays named in the tasks must stay valid while the batch is used; the
+ * launch reads whatever they hold when it runs, so a caller refills inputs
+ * and runs again (also from a captured hipGraph: vg_batch_run enqueues one
+ * kernel and nothing else).
+ */
+typedef struct vg_batch vg_batch;
+int vg_batch_create(const vg_geometry *g, const vg_query_task *tasks, int32_t ntasks, vg_batch **out);
+int vg_batch_run(vg_batch *b, void *stream);
+/* Launch shape: items per lane (1, 2, 4) and the number of persistent
+ * workgroups (0 = one workgroup per tile).  Synchronises the device. */
+int vg_batch_set_launch(vg_batch *b, int32_t items_per_lane, uint32_t blocks);
+int vg_batch_get_launch(const vg_batch *b, int32_t *items_per_lane, uint32_t *blocks, uint64_t *items);
+/* Times every launch shape on the batch itself (`repetitions` runs each) and
+ * keeps the fastest; *best_ms (nullable) receives its time per run.  Runs the
+ * batch, so outputs are written; synchronises the device. */
+int vg_batch_autotune(vg_batch *b, int32_t repetitions, void *stream, float *best_ms);
+int vg_batch_free(vg_batch *b);
+
 /* -------------------------------------------------------------- navigation */
 /*
  * Navigation states of a basket of tracks, structure-of-arrays in device

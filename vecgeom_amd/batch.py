@@ -1,0 +1,126 @@
+"""Tasks, batches and FP32-storage queries through the C-ABI.
+
+A task is one basket query on one placed volume (VPlacedVolume SOA3D
+virtuals, volumes/PlacedVolume.h:147-246) whose inputs may be float64 (the
+FP64 form) or float32 (FP32 storage, promoted to double before any
+arithmetic).  A Batch runs a fixed table of tasks -- any mix of shapes,
+queries and basket sizes -- in one kernel launch per shape weight class, and
+can autotune its launch shape.  Work goes on torch's current stream.
+"""
+import ctypes
+
+import torch
+
+from . import capi
+
+CONTAINS, INSIDE, DISTANCE_TO_IN, DISTANCE_TO_OUT, PLACED_DISTANCE_TO_OUT, SAFETY_TO_IN, SAFETY_TO_OUT = range(7)
+F64, F32 = 0, 1
+OUT_DTYPE = {CONTAINS: torch.uint8, INSIDE: torch.int32}
+DIRECTIONAL = {DISTANCE_TO_IN, DISTANCE_TO_OUT, PLACED_DISTANCE_TO_OUT}
+
+c_vp = ctypes.c_void_p
+
+
+class CQueryTask(ctypes.Structure):
+    _fields_ = [
+        ("placed", ctypes.c_int32),
+        ("query", ctypes.c_int32),
+        ("input_type", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("n", ctypes.c_uint64),
+        ("x", c_vp), ("y", c_vp), ("z", c_vp),
+        ("dx", c_vp), ("dy", c_vp), ("dz", c_vp),
+        ("step_max", c_vp),
+        ("out", c_vp),
+    ]
+
+
+_T = ctypes.POINTER(CQueryTask)
+capi.register({
+    "vg_query": [c_vp, _T, c_vp],
+    "vg_batch_create": [c_vp, _T, ctypes.c_int32, ctypes.POINTER(c_vp)],
+    "vg_batch_run": [c_vp, c_vp],
+    "vg_batch_set_launch": [c_vp, ctypes.c_int32, ctypes.c_uint32],
+    "vg_batch_get_launch": [c_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_uint32),
+                            ctypes.POINTER(ctypes.c_uint64)],
+    "vg_batch_autotune": [c_vp, ctypes.c_int32, c_vp, ctypes.POINTER(ctypes.c_float)],
+    "vg_batch_free": [c_vp],
+})
+
+
+def _stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+class Task:
+    """One query: placed id, query code, point triple, optional direction
+    triple and step_max, output tensor (allocated if None)."""
+
+    def __init__(self, placed, query, p, d=None, step_max=None, out=None):
+        if len(p) != 3 or (query in DIRECTIONAL and (d is None or len(d) != 3)):
+            raise ValueError("points (and directions for distance queries) must be (x, y, z) triples")
+        dt = p[0].dtype
+        if dt not in (torch.float64, torch.float32):
+            raise ValueError("inputs must be float64 or float32")
+        n = p[0].numel()
+        for a in list(p) + (list(d) if d is not None else []):
+            if a.dtype != dt or a.numel() != n or not a.is_cuda or not a.is_contiguous():
+                raise ValueError("inputs must be contiguous CUDA tensors of one dtype and length")
+        if step_max is not None and (step_max.dtype != torch.float64 or step_max.numel() != n):
+            raise ValueError("step_max must be float64 with one entry per point")
+        if out is None:
+            out = torch.empty(n, dtype=OUT_DTYPE.get(query, torch.float64), device=p[0].device)
+        self.placed, self.query, self.p, self.d, self.step_max, self.out, self.n = placed, query, p, d, step_max, out, n
+        self.input_type = F32 if dt == torch.float32 else F64
+
+    def c(self):
+        d = self.d if self.d is not None else (None, None, None)
+        ptr = lambda a: None if a is None else a.data_ptr()
+        return CQueryTask(int(self.placed), int(self.query), self.input_type, 0, self.n,
+                          *[ptr(a) for a in self.p], *[ptr(a) for a in d], ptr(self.step_max), self.out.data_ptr())
+
+
+def query(geom, task):
+    """Runs one task (any query, float64 or float32 inputs); returns its out."""
+    c = task.c()
+    capi.call("vg_query", geom.handle, ctypes.byref(c), _stream())
+    return task.out
+
+
+class Batch:
+    """A fixed table of tasks run together (vg_batch_*)."""
+
+    def __init__(self, geom, tasks):
+        self.geom = geom
+        self.tasks = list(tasks)  # keeps the tensors alive
+        arr = (CQueryTask * max(1, len(self.tasks)))(*[t.c() for t in self.tasks])
+        h = c_vp()
+        capi.call("vg_batch_create", geom.handle, arr, len(self.tasks), ctypes.byref(h))
+        self.handle = h
+
+    def run(self):
+        capi.call("vg_batch_run", self.handle, _stream())
+
+    def set_launch(self, items_per_lane, blocks=0):
+        capi.call("vg_batch_set_launch", self.handle, int(items_per_lane), int(blocks))
+
+    def launch(self):
+        it, bl, items = ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_uint64()
+        capi.call("vg_batch_get_launch", self.handle, ctypes.byref(it), ctypes.byref(bl), ctypes.byref(items))
+        return it.value, bl.value, items.value
+
+    def autotune(self, repetitions=3):
+        ms = ctypes.c_float()
+        capi.call("vg_batch_autotune", self.handle, int(repetitions), _stream(), ctypes.byref(ms))
+        return ms.value
+
+    def free(self):
+        if self.handle:
+            capi.call("vg_batch_free", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -40,6 +40,56 @@ struct PolyconeS {
     return -2;
   }
 
+  // False only when the cone of section i would return kInfinity from
+  // DistanceToIn: every finite result of the cone is the distance to a point
+  // of the ray within its z slab and between its inner and outer radii (hit
+  // points on its surfaces, or the start point itself when on a surface), so
+  // a ray that stays more than kPad away from the slab-bounded annulus
+  // min(rmin) <= rho <= max(rmax) cannot get one.  kPad (1e-6 mm) exceeds
+  // every tolerance of the cone and the rounding of this test.
+  static VGA_HD bool RayNearSection(const P &q, int i, V3 p, V3 v) {
+    constexpr double kPad = 1e-6;
+    const ConeP &c = Sec(q, i);
+    const double hz = c.dz + kPad;
+    const double pz = p.z - Shift(q, i);
+    double t0 = 0., t1 = kInfinity;
+    if (v.z != 0) {
+      const double inv = 1. / v.z;
+      double ta = (-hz - pz) * inv, tb = (hz - pz) * inv;
+      if (ta > tb) {
+        const double t = ta;
+        ta = tb;
+        tb = t;
+      }
+      t0 = ta > t0 ? ta : t0;
+      t1 = tb;
+      if (!(t0 <= t1)) return false;
+    } else if (Abs(pz) > hz) {
+      return false;
+    }
+    const double a = v.x * v.x + v.y * v.y, b = p.x * v.x + p.y * v.y;
+    double t = a > 0 ? -b / a : t0;
+    t = t < t0 ? t0 : (t > t1 ? t1 : t);
+    const double x = p.x + t * v.x, y = p.y + t * v.y;
+    const double rOut = (c.rmax1 > c.rmax2 ? c.rmax1 : c.rmax2) + kPad;
+    if (x * x + y * y > rOut * rOut) return false;
+    const double rIn = (c.rmin1 < c.rmin2 ? c.rmin1 : c.rmin2) - kPad;
+    if (rIn > 0 && t1 < kInfinity) {
+      // rho^2 is convex along the ray: its maximum on [t0, t1] is at an end.
+      const double x0 = p.x + t0 * v.x, y0 = p.y + t0 * v.y, x1 = p.x + t1 * v.x, y1 = p.y + t1 * v.y;
+      const double r0 = x0 * x0 + y0 * y0, r1 = x1 * x1 + y1 * y1;
+      if ((r0 > r1 ? r0 : r1) < rIn * rIn) return false;
+    }
+    return true;
+  }
+
+  // The z term of the section's cone SafetyToIn, evaluated as the cone does:
+  // its safety is at least this, so a section whose z term already reaches
+  // the running minimum cannot lower it.
+  static VGA_HD double SectionSafeZ(const P &q, int i, V3 p) {
+    return Abs(Local(p, Shift(q, i)).z) - Sec(q, i).dz;
+  }
+
   static VGA_HD bool Contains(const P &q, V3 p) {
     const int n = NSec(q);
     if (p.z < Z(q, 0) || p.z > Z(q, n)) return false;
@@ -58,7 +108,9 @@ struct PolyconeS {
     if (index == -1) index = 0;
     if (index == -2) index = n - 1;
     do {
-      distance = ConeS::DistanceToIn(Sec(q, index), Local(p, Shift(q, index)), v, stepMax);
+      distance = RayNearSection(q, index, p, v)
+                     ? ConeS::DistanceToIn(Sec(q, index), Local(p, Shift(q, index)), v, stepMax)
+                     : kInfinity;
       if (distance < kInfinity || !increment) break;
       index += increment;
     } while (index >= 0 && index < n);
@@ -130,6 +182,7 @@ struct PolyconeS {
     double zbase = Z(q, index + 1);
     for (int i = index + 1; i < n; ++i) {
       if (Z(q, i) - zbase >= minSafety) break;
+      if (SectionSafeZ(q, i, p) >= minSafety) continue;
       safety = ConeS::SafetyToInImpl(Sec(q, i), Local(p, Shift(q, i)), false);
       if (safety < minSafety) minSafety = safety;
     }
@@ -137,6 +190,7 @@ struct PolyconeS {
       zbase = Z(q, index - 1);
       for (int i = index - 1; i >= 0; --i) {
         if (zbase - Z(q, i) >= minSafety) break;
+        if (SectionSafeZ(q, i, p) >= minSafety) continue;
         safety = ConeS::SafetyToInImpl(Sec(q, i), Local(p, Shift(q, i)), false);
         if (safety < minSafety) minSafety = safety;
       }
@@ -154,6 +208,7 @@ struct PolyconeS {
     double zbase = Z(q, index + 1);
     for (int i = index + 1; i < n; ++i) {
       if (Z(q, i) - zbase >= minSafety) break;
+      if (SectionSafeZ(q, i, p) >= minSafety) continue;
       const double safety = ConeS::SafetyToInImpl(Sec(q, i), Local(p, Shift(q, i)), false);
       if (safety < minSafety) minSafety = safety;
     }
@@ -161,6 +216,7 @@ struct PolyconeS {
       zbase = Z(q, index - 1);
       for (int i = index - 1; i >= 0; --i) {
         if (zbase - Z(q, i) >= minSafety) break;
+        if (SectionSafeZ(q, i, p) >= minSafety) continue;
         const double safety = ConeS::SafetyToInImpl(Sec(q, i), Local(p, Shift(q, i)), false);
         if (safety < minSafety) minSafety = safety;
       }

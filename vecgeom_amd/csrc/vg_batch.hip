@@ -1,3 +1,519 @@
-// Batched multi-shape launches (filled in with the batching layer).
+// Batched multi-shape query launch for gfx950.
+//
+// A batch is a table of tasks, each one basket query on one placed volume
+// (the per-shape work queues: any mix of shapes, queries and basket sizes).
+// The tasks are cut into tiles of kBatchBlock * IT items; one launch covers
+// every tile of every task, so a step of many small and large baskets costs
+// one dispatch instead of one per (shape, query) and no launch leaves the
+// chip idle at its tail while the next waits.  Each tile belongs to a single
+// task, so every lane of a workgroup evaluates the same shape and query: the
+// task record, the shape parameters and the placement are uniform per tile
+// (scalar loads), and the switch over shape and query is a uniform branch.
+//
+// Shapes are split into three weight classes by the registers their kernels
+// need (light: Box, Tube, Orb, Trd, Trapezoid, Parallelepiped, Paraboloid;
+// medium: Cone, Sphere, Polycone; heavy: Torus).  Each class has its own
+// kernel, so a Box tile is not held to the register budget of the Torus
+// quartic; a batch costs one launch per class it uses.
+//
+// Launch shapes: grid = one workgroup per tile, or a persistent grid of
+// `blocks` workgroups striding over the tiles; IT = 1, 2 or 4 items per lane.
+// vg_batch_autotune picks both by timing the batch itself.
+//
+// Inputs may be double (the FP64 form) or float (FP32 storage: each value is
+// promoted to double in registers, so results equal the FP64 kernels on the
+// same float-valued inputs; distances keep FP64 arithmetic, which the
+// 1e-9 mm bar requires -- one float ulp at 50 mm is 4e-6 mm).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "shapes.hpp"
 #include "vg_internal.hpp"
+
+namespace vga {
+
+namespace {
+
+constexpr int kBatchBlock = 256;
+
+struct TaskDev {
+  int32_t type, query, xfkind, in_type;
+  int32_t param_off, vec_ok;
+  uint64_t n, tile_begin;
+  const void *x, *y, *z, *dx, *dy, *dz;
+  const double *sm;
+  void *out;
+  Xf xf;
+};
+
+template <class T, int IT>
+struct alignas(sizeof(T) * IT) Vec {
+  T v[IT];
+};
+
+template <class T, int IT>
+__device__ __forceinline__ void Load(const void *base, uint64_t i0, bool full, bool vec, uint64_t n, double (&o)[IT]) {
+  const T *p = static_cast<const T *>(base) + i0;
+  if (full && vec) {
+    const Vec<T, IT> t = *reinterpret_cast<const Vec<T, IT> *>(p);
+#pragma unroll
+    for (int k = 0; k < IT; ++k) o[k] = (double)t.v[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < IT; ++k) o[k] = (i0 + k < n) ? (double)p[k] : 0.;
+  }
+}
+
+template <int IT>
+__device__ __forceinline__ void LoadAll(const TaskDev t, uint64_t i0, bool full, const void *a, double (&o)[IT]) {
+  const bool vec = t.vec_ok != 0;
+  if (t.in_type == VG_F32)
+    Load<float, IT>(a, i0, full, vec, t.n, o);
+  else
+    Load<double, IT>(a, i0, full, vec, t.n, o);
+}
+
+template <class S, int Q, int IT>
+__device__ __forceinline__ void RunTile(const TaskDev t, const double *params, uint64_t i0) {
+  const auto &P = ParamView<S>::Get(params + t.param_off);
+  const bool full = i0 + IT <= t.n;
+  constexpr bool dir = (Q == kQDistIn || Q == kQDistOut || Q == kQPlacedDistOut);
+  double px[IT], py[IT], pz[IT], dx[IT], dy[IT], dz[IT], sm[IT];
+  LoadAll<IT>(t, i0, full, t.x, px);
+  LoadAll<IT>(t, i0, full, t.y, py);
+  LoadAll<IT>(t, i0, full, t.z, pz);
+  if constexpr (dir) {
+    LoadAll<IT>(t, i0, full, t.dx, dx);
+    LoadAll<IT>(t, i0, full, t.dy, dy);
+    LoadAll<IT>(t, i0, full, t.dz, dz);
+#pragma unroll
+    for (int k = 0; k < IT; ++k) sm[k] = (t.sm && i0 + k < t.n) ? t.sm[i0 + k] : kInfinity;
+  }
+#pragma unroll
+  for (int k = 0; k < IT; ++k) {
+    if (i0 + k >= t.n) break;
+    const V3 p{px[k], py[k], pz[k]};
+    const uint64_t i = i0 + k;
+    if constexpr (Q == kQContains) {
+      static_cast<uint8_t *>(t.out)[i] = S::Contains(P, XfPointRt(t.xf, t.xfkind, p)) ? 1 : 0;
+    } else if constexpr (Q == kQInside) {
+      static_cast<int32_t *>(t.out)[i] = S::Inside(P, XfPointRt(t.xf, t.xfkind, p));
+    } else if constexpr (Q == kQDistIn) {
+      const V3 d{dx[k], dy[k], dz[k]};
+      static_cast<double *>(t.out)[i] =
+          S::DistanceToIn(P, XfPointRt(t.xf, t.xfkind, p), XfDirRt(t.xf, t.xfkind, d), sm[k]);
+    } else if constexpr (Q == kQDistOut) {
+      static_cast<double *>(t.out)[i] = S::DistanceToOut(P, p, V3{dx[k], dy[k], dz[k]}, sm[k]);
+    } else if constexpr (Q == kQPlacedDistOut) {
+      const V3 d{dx[k], dy[k], dz[k]};
+      static_cast<double *>(t.out)[i] =
+          S::DistanceToOut(P, XfPointRt(t.xf, t.xfkind, p), XfDirRt(t.xf, t.xfkind, d), sm[k]);
+    } else if constexpr (Q == kQSafeIn) {
+      static_cast<double *>(t.out)[i] = S::SafetyToIn(P, XfPointRt(t.xf, t.xfkind, p));
+    } else {
+      static_cast<double *>(t.out)[i] = S::SafetyToOut(P, p);
+    }
+  }
+}
+
+template <class S, int IT>
+__device__ __forceinline__ void RunShape(const TaskDev t, const double *params, uint64_t i0) {
+  switch (t.query) {
+    case kQContains: RunTile<S, kQContains, IT>(t, params, i0); break;
+    case kQInside: RunTile<S, kQInside, IT>(t, params, i0); break;
+    case kQDistIn: RunTile<S, kQDistIn, IT>(t, params, i0); break;
+    case kQDistOut: RunTile<S, kQDistOut, IT>(t, params, i0); break;
+    case kQPlacedDistOut: RunTile<S, kQPlacedDistOut, IT>(t, params, i0); break;
+    case kQSafeIn: RunTile<S, kQSafeIn, IT>(t, params, i0); break;
+    case kQSafeOut: RunTile<S, kQSafeOut, IT>(t, params, i0); break;
+  }
+}
+
+constexpr int kClasses = 3;
+
+constexpr int ClassOf(int type) {
+  return (type == VG_CONE || type == VG_SPHERE || type == VG_POLYCONE) ? 1 : (type == VG_TORUS ? 2 : 0);
+}
+
+template <int IT, int CLS>
+__device__ __forceinline__ void RunTask(const TaskDev t, const double *params, uint64_t i0) {
+  if constexpr (CLS == 0) {
+    switch (t.type) {
+      case VG_BOX: RunShape<BoxS, IT>(t, params, i0); break;
+      case VG_TUBE: RunShape<TubeS, IT>(t, params, i0); break;
+      case VG_ORB: RunShape<OrbS, IT>(t, params, i0); break;
+      case VG_TRD: RunShape<TrdS, IT>(t, params, i0); break;
+      case VG_TRAPEZOID: RunShape<TrapS, IT>(t, params, i0); break;
+      case VG_PARALLELEPIPED: RunShape<ParaS, IT>(t, params, i0); break;
+      case VG_PARABOLOID: RunShape<ParabS, IT>(t, params, i0); break;
+    }
+  } else if constexpr (CLS == 1) {
+    switch (t.type) {
+      case VG_SPHERE: RunShape<SphereS, IT>(t, params, i0); break;
+      case VG_CONE: RunShape<ConeS, IT>(t, params, i0); break;
+      case VG_POLYCONE: RunShape<PolyconeS, IT>(t, params, i0); break;
+    }
+  } else {
+    if (t.type == VG_TORUS) RunShape<TorusS, IT>(t, params, i0);
+  }
+}
+
+// Tile -> task: the last task whose first tile is <= tile (tasks sorted by
+// tile_begin; every task has at least one tile).
+__device__ __forceinline__ int FindTask(const TaskDev *tasks, int ntasks, uint64_t tile) {
+  int lo = 0, hi = ntasks - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].tile_begin <= tile)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int IT, int CLS>
+__global__ __launch_bounds__(kBatchBlock) void BatchKernel(const TaskDev *__restrict__ tasks, int ntasks,
+                                                           uint64_t ntiles, const double *__restrict__ params) {
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int k = FindTask(tasks, ntasks, tile);
+    const TaskDev t = tasks[k];
+    const uint64_t i0 = ((tile - t.tile_begin) * kBatchBlock + threadIdx.x) * IT;
+    if (i0 < t.n) RunTask<IT, CLS>(t, params, i0);
+  }
+}
+
+// One task passed by value (kernel argument): the single-query form of the
+// same code, for callers that do not keep a batch.
+template <int IT, int CLS>
+__global__ __launch_bounds__(kBatchBlock) void TaskKernel(const TaskDev t, const double *__restrict__ params) {
+  const uint64_t i0 = ((uint64_t)blockIdx.x * kBatchBlock + threadIdx.x) * IT;
+  if (i0 < t.n) RunTask<IT, CLS>(t, params, i0);
+}
+
+}  // namespace
+
+}  // namespace vga
+
+using namespace vga;
+
+struct vg_batch {
+  const vg_geometry *g = nullptr;
+  std::vector<vg_query_task> tasks;
+  std::vector<TaskDev> host;                  // sorted by weight class
+  int first[vga::kClasses + 1] = {0, 0, 0, 0};  // class c owns host[first[c] .. first[c+1])
+  uint64_t ntiles[vga::kClasses] = {0, 0, 0};   // tiles per class for the current IT
+  TaskDev *d_tasks = nullptr;
+  int it = 2;            // items per lane
+  unsigned blocks = 0;   // 0 = one workgroup per tile, else a persistent grid per class
+  uint64_t items = 0;
+  // Classes run concurrently: the first non-empty class on the caller's
+  // stream, the others on side streams forked from and joined back to it
+  // with events (a pattern hipGraph capture records as parallel branches).
+  hipStream_t side[vga::kClasses - 1] = {nullptr, nullptr};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[vga::kClasses - 1] = {nullptr, nullptr};
+};
+
+namespace {
+
+constexpr int kITs[3] = {1, 2, 4};
+
+bool Aligned(const void *p, size_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
+
+int QueryCode(int32_t q) {
+  switch (q) {
+    case VG_Q_CONTAINS: return kQContains;
+    case VG_Q_INSIDE: return kQInside;
+    case VG_Q_DISTANCE_TO_IN: return kQDistIn;
+    case VG_Q_DISTANCE_TO_OUT: return kQDistOut;
+    case VG_Q_PLACED_DISTANCE_TO_OUT: return kQPlacedDistOut;
+    case VG_Q_SAFETY_TO_IN: return kQSafeIn;
+    case VG_Q_SAFETY_TO_OUT: return kQSafeOut;
+  }
+  return -1;
+}
+
+// Fills tile_begin (numbered within each class) for items per lane `it`.
+void Schedule(vg_batch *b) {
+  for (int c = 0; c < kClasses; ++c) {
+    uint64_t tile = 0;
+    for (int k = b->first[c]; k < b->first[c + 1]; ++k) {
+      TaskDev &x = b->host[k];
+      x.tile_begin = tile;
+      tile += (x.n + (uint64_t)kBatchBlock * b->it - 1) / ((uint64_t)kBatchBlock * b->it);
+    }
+    b->ntiles[c] = tile;
+  }
+}
+
+template <int IT, int CLS>
+void LaunchClass(const vg_batch *b, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL((BatchKernel<IT, CLS>), dim3(grid), dim3(kBatchBlock), 0, s, b->d_tasks + b->first[CLS],
+                     b->first[CLS + 1] - b->first[CLS], b->ntiles[CLS], b->g->d_params);
+}
+
+template <int CLS>
+void LaunchIT(const vg_batch *b, unsigned grid, hipStream_t s) {
+  if (b->it == 1)
+    LaunchClass<1, CLS>(b, grid, s);
+  else if (b->it == 2)
+    LaunchClass<2, CLS>(b, grid, s);
+  else
+    LaunchClass<4, CLS>(b, grid, s);
+}
+
+int Launch(vg_batch *b, hipStream_t s) {
+  int classes = 0;
+  for (int c = 0; c < kClasses; ++c) classes += b->ntiles[c] ? 1 : 0;
+  int rc = VG_OK;
+  if (classes > 1) rc = CheckHip(hipEventRecord(b->fork, s), "batch fork");
+  int nside = 0;
+  bool first = true;
+  for (int c = 0; c < kClasses && !rc; ++c) {
+    const uint64_t nt = b->ntiles[c];
+    if (nt == 0) continue;
+    hipStream_t cs = s;
+    if (!first) {
+      // later classes: a side stream that waits for the fork
+      cs = b->side[nside];
+      rc = CheckHip(hipStreamWaitEvent(cs, b->fork, 0), "batch fork wait");
+      if (rc) break;
+    }
+    first = false;
+    const unsigned grid = b->blocks ? (unsigned)std::min<uint64_t>(b->blocks, nt) : (unsigned)nt;
+    if (c == 0)
+      LaunchIT<0>(b, grid, cs);
+    else if (c == 1)
+      LaunchIT<1>(b, grid, cs);
+    else
+      LaunchIT<2>(b, grid, cs);
+    rc = CheckHip(hipGetLastError(), "batch launch");
+    if (!rc && cs != s) {
+      rc = CheckHip(hipEventRecord(b->join[nside], cs), "batch join");
+      if (!rc) rc = CheckHip(hipStreamWaitEvent(s, b->join[nside], 0), "batch join wait");
+      ++nside;
+    }
+  }
+  return rc;
+}
+
+void DestroySync(vg_batch *b) {
+  for (auto &x : b->side)
+    if (x) (void)hipStreamDestroy(x);
+  for (auto &x : b->join)
+    if (x) (void)hipEventDestroy(x);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+}
+
+int CreateSync(vg_batch *b) {
+  int rc = CheckHip(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "batch event");
+  for (int k = 0; k < kClasses - 1 && !rc; ++k) {
+    rc = CheckHip(hipStreamCreateWithFlags(&b->side[k], hipStreamNonBlocking), "batch stream");
+    if (!rc) rc = CheckHip(hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming), "batch event");
+  }
+  return rc;
+}
+
+// Rewrites the device task table for the current IT.  Launches already
+// enqueued may still read the old table, so the device is drained first.
+int Upload(vg_batch *b) {
+  Schedule(b);
+  if (b->host.empty()) return VG_OK;
+  int rc = CheckHip(hipDeviceSynchronize(), "batch table: device synchronize");
+  if (rc) return rc;
+  return CheckHip(hipMemcpy(b->d_tasks, b->host.data(), b->host.size() * sizeof(TaskDev), hipMemcpyHostToDevice),
+                  "batch table upload");
+}
+
+// Validates one task and fills its device record; returns a status.
+int MakeTask(const vg_geometry *g, const vg_query_task &q, int i, TaskDev &t) {
+  if (q.placed < 0 || (size_t)q.placed >= g->placed.size())
+    return SetError(VG_ERR_INVALID, "task " + std::to_string(i) + ": bad placed id");
+  const PlacedRec &pr = g->placed[q.placed];
+  if (!ShapeHasKernels(pr.type))
+    return SetError(VG_ERR_UNSUPPORTED, "task " + std::to_string(i) + ": no device kernels for the shape");
+  const int qc = QueryCode(q.query);
+  if (qc < 0 || (q.input_type != VG_F64 && q.input_type != VG_F32))
+    return SetError(VG_ERR_INVALID, "task " + std::to_string(i) + ": bad query or input type");
+  const bool dir = qc == kQDistIn || qc == kQDistOut || qc == kQPlacedDistOut;
+  if (q.n > 0 && (!q.x || !q.y || !q.z || !q.out || (dir && (!q.dx || !q.dy || !q.dz))))
+    return SetError(VG_ERR_INVALID, "task " + std::to_string(i) + ": null array");
+  std::memset(&t, 0, sizeof(t));
+  t.type = pr.type;
+  t.query = qc;
+  t.xfkind = pr.xfkind;
+  t.in_type = q.input_type;
+  t.param_off = pr.param_off;
+  t.n = q.n;
+  t.x = q.x;
+  t.y = q.y;
+  t.z = q.z;
+  t.dx = q.dx;
+  t.dy = q.dy;
+  t.dz = q.dz;
+  t.sm = q.step_max;
+  t.out = q.out;
+  t.xf = pr.xf;
+  // Vector loads need every input array aligned to a whole vector of the
+  // largest IT (4 items).
+  const size_t a = (t.in_type == VG_F32 ? 4 : 8) * 4;
+  bool ok = Aligned(t.x, a) && Aligned(t.y, a) && Aligned(t.z, a);
+  if (dir) ok = ok && Aligned(t.dx, a) && Aligned(t.dy, a) && Aligned(t.dz, a);
+  t.vec_ok = ok ? 1 : 0;
+  return VG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vg_batch_create(const vg_geometry *g, const vg_query_task *tasks, int32_t ntasks, vg_batch **out) {
+  if (!g || !out || ntasks < 0 || (ntasks > 0 && !tasks)) return SetError(VG_ERR_INVALID, "null argument");
+  *out = nullptr;
+  int rc = EnsureDevice();
+  if (rc) return rc;
+  auto *b = new vg_batch;
+  b->g = g;
+  for (int i = 0; i < ntasks; ++i) {
+    TaskDev t;
+    rc = MakeTask(g, tasks[i], i, t);
+    if (rc) {
+      delete b;
+      return rc;
+    }
+    b->tasks.push_back(tasks[i]);
+    if (t.n == 0) continue;
+    b->items += t.n;
+    b->host.push_back(t);
+  }
+  std::stable_sort(b->host.begin(), b->host.end(),
+                   [](const TaskDev &a, const TaskDev &c) { return ClassOf(a.type) < ClassOf(c.type); });
+  for (int c = 0; c <= kClasses; ++c) {
+    int k = 0;
+    while (k < (int)b->host.size() && ClassOf(b->host[k].type) < c) ++k;
+    b->first[c] = k;
+  }
+  rc = CreateSync(b);
+  if (!rc && !b->host.empty()) {
+    rc = CheckHip(hipMalloc(&b->d_tasks, b->host.size() * sizeof(TaskDev)), "hipMalloc(batch table)");
+    if (!rc) rc = Upload(b);
+  }
+  if (rc) {
+    if (b->d_tasks) (void)hipFree(b->d_tasks);
+    DestroySync(b);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return VG_OK;
+}
+
+int vg_query(const vg_geometry *g, const vg_query_task *task, void *stream) {
+  if (!g || !task) return SetError(VG_ERR_INVALID, "null argument");
+  int rc = EnsureDevice();
+  if (rc) return rc;
+  TaskDev t;
+  if ((rc = MakeTask(g, *task, 0, t))) return rc;
+  if (t.n == 0) return VG_OK;
+  const int it = t.vec_ok ? 2 : 1;
+  const uint64_t blocks = (t.n + (uint64_t)it * kBatchBlock - 1) / ((uint64_t)it * kBatchBlock);
+  const hipStream_t s = (hipStream_t)stream;
+  const dim3 gr((unsigned)blocks), bl(kBatchBlock);
+  switch (ClassOf(t.type) * 2 + (it == 2 ? 1 : 0)) {
+    case 0: hipLaunchKernelGGL((TaskKernel<1, 0>), gr, bl, 0, s, t, g->d_params); break;
+    case 1: hipLaunchKernelGGL((TaskKernel<2, 0>), gr, bl, 0, s, t, g->d_params); break;
+    case 2: hipLaunchKernelGGL((TaskKernel<1, 1>), gr, bl, 0, s, t, g->d_params); break;
+    case 3: hipLaunchKernelGGL((TaskKernel<2, 1>), gr, bl, 0, s, t, g->d_params); break;
+    case 4: hipLaunchKernelGGL((TaskKernel<1, 2>), gr, bl, 0, s, t, g->d_params); break;
+    default: hipLaunchKernelGGL((TaskKernel<2, 2>), gr, bl, 0, s, t, g->d_params); break;
+  }
+  return CheckHip(hipGetLastError(), "query launch");
+}
+
+int vg_batch_run(vg_batch *b, void *stream) {
+  if (!b) return SetError(VG_ERR_INVALID, "null batch");
+  int rc = EnsureDevice();
+  if (rc) return rc;
+  return Launch(b, (hipStream_t)stream);
+}
+
+int vg_batch_set_launch(vg_batch *b, int32_t items_per_lane, uint32_t blocks) {
+  if (!b) return SetError(VG_ERR_INVALID, "null batch");
+  if (items_per_lane != 1 && items_per_lane != 2 && items_per_lane != 4)
+    return SetError(VG_ERR_INVALID, "items per lane must be 1, 2 or 4");
+  int rc = EnsureDevice();
+  if (rc) return rc;
+  b->it = items_per_lane;
+  b->blocks = blocks;
+  return Upload(b);
+}
+
+int vg_batch_get_launch(const vg_batch *b, int32_t *items_per_lane, uint32_t *blocks, uint64_t *items) {
+  if (!b) return SetError(VG_ERR_INVALID, "null batch");
+  if (items_per_lane) *items_per_lane = b->it;
+  if (blocks) *blocks = b->blocks;
+  if (items) *items = b->items;
+  return VG_OK;
+}
+
+int vg_batch_autotune(vg_batch *b, int32_t repetitions, void *stream, float *best_ms) {
+  if (!b || repetitions < 1) return SetError(VG_ERR_INVALID, "null batch or repetitions < 1");
+  int rc = EnsureDevice();
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const unsigned grids[5] = {0u, (unsigned)cus * 2, (unsigned)cus * 4, (unsigned)cus * 8, (unsigned)cus * 16};
+  hipEvent_t e0, e1;
+  if ((rc = CheckHip(hipEventCreate(&e0), "hipEventCreate"))) return rc;
+  if ((rc = CheckHip(hipEventCreate(&e1), "hipEventCreate"))) {
+    (void)hipEventDestroy(e0);
+    return rc;
+  }
+  float best = 1e30f;
+  int bestIt = b->it;
+  unsigned bestBlocks = b->blocks;
+  for (int it : kITs) {
+    for (unsigned gr : grids) {
+      b->it = it;
+      b->blocks = gr;
+      if ((rc = Upload(b))) break;
+      if ((rc = Launch(b, s))) break;  // warm-up
+      (void)hipEventRecord(e0, s);
+      for (int r = 0; r < repetitions && !rc; ++r) rc = Launch(b, s);
+      (void)hipEventRecord(e1, s);
+      if (rc || (rc = CheckHip(hipEventSynchronize(e1), "autotune sync"))) break;
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      ms /= repetitions;
+      if (ms < best) {
+        best = ms;
+        bestIt = it;
+        bestBlocks = gr;
+      }
+    }
+    if (rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->it = bestIt;
+  b->blocks = bestBlocks;
+  int rc2 = Upload(b);
+  if (best_ms) *best_ms = best;
+  return rc ? rc : rc2;
+}
+
+int vg_batch_free(vg_batch *b) {
+  if (!b) return SetError(VG_ERR_INVALID, "null batch");
+  if (b->d_tasks) (void)hipFree(b->d_tasks);
+  DestroySync(b);
+  delete b;
+  return VG_OK;
+}
+
+}  // extern "C"
