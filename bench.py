@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--nav-tracks", type=int, default=1 << 20, help="tracks of the nested-geometry navigation run")
     ap.add_argument("--nav-steps", type=int, default=3, help="timed trace launches")
     ap.add_argument("--no-nav", action="store_true")
+    ap.add_argument("--serve-baskets", type=int, default=2000, help="baskets per serving run")
+    ap.add_argument("--serve-steps", type=int, default=8, help="navigation steps per served basket")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a device-to-device copy of MIB MiB (for HBM counter scaling)")
     return ap.parse_args()
@@ -344,6 +347,63 @@ def nav_bench(args, dev, stream, with_cpu):
     return res
 
 
+def serve_bench(args, dev):
+    """Serving on the nested geometry: baskets with sizes log-uniform in
+    16..65536 tracks (vertex/direction distribution of the navigation run)
+    arrive as a seeded Poisson process, are admitted by footprint into a
+    32 GiB track arena, take `serve_steps` navigation steps each (one request
+    per step, the next submitted when the previous completes) and close.
+    Latency is per request, submission to completion.  The batched form
+    merges all pending requests into one launch on each of two streams; the
+    per-basket form launches each request alone, one at a time.  Rates are
+    fractions of the per-basket form's saturation rate (all baskets arriving
+    at once)."""
+    from vecgeom_amd import geometry as G
+    from vecgeom_amd import serve as S
+    from vecgeom_amd import workloads as W
+
+    _, flat = W.nested_geometry()
+    geom = G.DeviceGeometry(flat)
+    rng = np.random.Generator(np.random.Philox(2024))
+    nb, steps = args.serve_baskets, args.serve_steps
+    sizes = np.exp(rng.uniform(np.log(16), np.log(65536), nb)).astype(np.uint64)
+    p, d = W.track_starts(int(sizes.sum()), seed=4040)
+    srv = S.Server(geom, 32 << 30)
+    forms = {"batched": (4096, 2), "per_basket": (1, 1)}
+
+    def run(form, arrival):
+        srv.set_batching(*forms[form])
+        s0 = srv.stats()
+        lat, wait, wall = srv.replay(arrival, sizes, steps, p, d)
+        s1 = srv.stats()
+        ts = s1["track_steps"] - s0["track_steps"]
+        req = s1["requests"] - s0["requests"]
+        la = lat.ravel() * 1e3
+        return {"p50_ms": round(float(np.percentile(la, 50)), 4), "p99_ms": round(float(np.percentile(la, 99)), 4),
+                "mean_ms": round(float(la.mean()), 4), "admit_wait_p99_ms": round(float(np.percentile(wait, 99) * 1e3), 4),
+                "steps_per_s": round(ts / wall / 1e9, 5), "wall_s": round(wall, 4),
+                "requests": req, "launches": s1["launches"] - s0["launches"]}
+
+    zero = np.zeros(nb)
+    run("batched", zero)  # warm-up
+    sat = {f: run(f, zero) for f in forms}
+    base_rate = nb / sat["per_basket"]["wall_s"]
+    rates = {}
+    for frac in (0.5, 0.9):
+        rate = frac * base_rate
+        gaps = rng.exponential(1.0 / rate, nb)
+        arrival = np.cumsum(gaps) - gaps[0]
+        rates[f"{rate:.0f}/s"] = {f: run(f, arrival) for f in forms}
+    res = {"workload": f"nested geometry, {nb} baskets, sizes log-uniform 16..65536 tracks "
+                       f"({int(sizes.sum())} tracks), {steps} steps per basket, Poisson arrivals, 32 GiB arena",
+           "unit_steps": "G steps/s", "saturation": sat,
+           "saturation_basket_rate_per_basket_form": round(base_rate, 1),
+           "fixed_rates": rates, "dtype": "f64"}
+    srv.destroy()
+    geom.free()
+    return res
+
+
 def nav_cpu_baseline(flat, p0, d0, target_s):
     """The reference's own ABBoxNavigator (built from its sources with its
     Release flags) on one core, on a bounded prefix of the same tracks; the
@@ -473,6 +533,11 @@ def main():
         log("navigation")
         nav = nav_bench(args, dev, stream, not args.no_cpu)
 
+    serve = None
+    if not args.no_serve:
+        log("serving")
+        serve = serve_bench(args, dev)
+
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gqueries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
@@ -489,6 +554,7 @@ def main():
         "plain_fp64_gpu": plain,
         "cpu_baseline": cpu,
         "nav": nav,
+        "serve": serve,
         "per_kernel": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()} for r in rows],
     }
     print(json.dumps(line), flush=True)

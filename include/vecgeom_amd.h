@@ -357,6 +357,74 @@ int vg_trace_tracks(const vg_geometry *g, int32_t navigator, const double *x, co
                     int32_t *nsteps, double *fx, double *fy, double *fz, double *travelled, uint64_t *hash,
                     int32_t *exact_steps, void *stream);
 
+/*
+ * ---------------------------------------------------------------- serving
+ * A navigation server with continuous batching.  The reference drives
+ * baskets of tracks through its navigators from its own event loop
+ * (test/globalbenchmarks/TraceTrack.cpp:170-220 steps one track at a time;
+ * the basket navigation entry points are navigation/NavigatorBase and
+ * NavStatePool).  Here a client opens a basket (its tracks become resident
+ * on the device together with their navigation states), then submits
+ * requests; each request is one step of the TraceTrack loop for every track
+ * of the basket: FindNextBoundaryAndStep with the server's pstep, then
+ * p += d * (step + 1e-6), plus the step count and the running hash of
+ * vg_trace_tracks.  A background thread merges all pending requests into one
+ * launch whenever one of two device streams is free, so requests that arrive
+ * while others run are batched into the next launch.  Requests for one
+ * basket run in submission order; N requests on a basket give exactly the
+ * state vg_trace_tracks reaches after N steps.
+ *
+ * Memory: the track arena is allocated once at create and sized by
+ * hbm_budget_bytes (8 * 8 + 3 * 4 + 4 * depth bytes per track); a basket is
+ * admitted only if a contiguous run of free slots holds it
+ * (VG_ERR_CAPACITY otherwise).  The geometry must outlive the server.
+ */
+typedef struct vg_server vg_server;
+
+int vg_server_create(const vg_geometry *g, uint64_t hbm_budget_bytes, int32_t navigator, double pstep,
+                     vg_server **out);
+/* Arena slots in total and currently free. */
+int vg_server_capacity(const vg_server *s, uint64_t *tracks_total, uint64_t *tracks_free);
+/* Uploads n tracks from host arrays and locates them from the world
+ * (synchronous); returns the basket id. */
+int vg_server_open(vg_server *s, uint64_t n, const double *x, const double *y, const double *z, const double *dx,
+                   const double *dy, const double *dz, int64_t *basket);
+/* Queues one navigation step of every track of the basket. */
+int vg_server_submit(vg_server *s, int64_t basket, int64_t *request);
+/* Blocks until the request has completed; latency_s (nullable) receives the
+ * time from submission to completion.  A request id is released by the wait
+ * (or by the poll) that reports it complete. */
+int vg_server_wait(vg_server *s, int64_t request, double *latency_s);
+/* Non-blocking: *done = 1 (and latency_s set) once complete. */
+int vg_server_poll(vg_server *s, int64_t request, int32_t *done, double *latency_s);
+/* Copies the basket's current track states to host arrays (each nullable;
+ * synchronous).  Fails with VG_ERR_STATE while a step of the basket runs. */
+int vg_server_read(vg_server *s, int64_t basket, double *x, double *y, double *z, int32_t *level, int32_t *boundary,
+                   int32_t *nsteps, uint64_t *hash, double *last_step);
+/* Frees the basket's slots; VG_ERR_STATE while it has queued or running steps. */
+int vg_server_close(vg_server *s, int64_t basket);
+/* Launches made, requests they carried, and track steps taken. */
+int vg_server_stats(vg_server *s, uint64_t *launches, uint64_t *requests_launched, uint64_t *track_steps);
+/* Launch policy: at most max_requests_per_launch requests (1..4096) merged
+ * into one launch, on 1 or 2 streams.  (1, 1) is the per-basket form: one
+ * launch per request, one at a time.  Default (4096, 2).  Fails with
+ * VG_ERR_STATE while requests are queued or running. */
+int vg_server_set_batching(vg_server *s, int32_t max_requests_per_launch, int32_t streams);
+/* In-process load generator (the transport client of a serving benchmark):
+ * basket i arrives arrival_s[i] seconds after the call (sorted), waits for
+ * admission by footprint (FIFO; VG_ERR_CAPACITY only if a basket exceeds the
+ * whole arena), is opened from its tracks -- the next sizes[i] entries of
+ * the concatenated x..dz arrays -- then takes steps_per_basket steps, each
+ * one request submitted when the previous completes, and is closed.
+ * latency_s[i * steps_per_basket + k] receives step k's latency;
+ * admit_wait_s[i] (nullable) the time from arrival to admission; wall_s
+ * (nullable) the elapsed time until the last basket closed. */
+int vg_server_replay(vg_server *s, uint64_t nbaskets, const double *arrival_s, const uint64_t *sizes,
+                     int32_t steps_per_basket, const double *x, const double *y, const double *z, const double *dx,
+                     const double *dy, const double *dz, double *latency_s, double *admit_wait_s, double *wall_s);
+/* Drains queued requests, stops the thread and frees the arena. */
+int vg_server_destroy(vg_server *s);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -36,5 +36,6 @@ def test_python_binding_covers_header():
     from vecgeom_amd import capi
     import vecgeom_amd.batch  # noqa: F401  (registers task and batch signatures)
     import vecgeom_amd.nav  # noqa: F401  (registers navigation signatures)
+    import vecgeom_amd.serve  # noqa: F401  (registers server signatures)
     missing = [n for n in declared() if n not in capi._SIGS]
     assert not missing, f"no ctypes signature for: {missing}"

@@ -10,6 +10,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libvecgeom_amd.so")
 
 VG_OK = 0
+VG_ERR_INVALID = -1
+VG_ERR_HIP = -2
+VG_ERR_STATE = -3
+VG_ERR_UNSUPPORTED = -4
+VG_ERR_CAPACITY = -5
 
 c_i32 = ctypes.c_int32
 c_sz = ctypes.c_size_t

@@ -22,7 +22,7 @@
 #else
 #define VGA_NAV_CHECK(cond, msg) ((void)0)
 #endif
-#define VGA_NAV __host__ __device__
+#define VGA_NAV inline __host__ __device__
 
 #include "shapes.hpp"
 #include "vg_internal.hpp"
