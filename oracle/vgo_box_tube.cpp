@@ -384,4 +384,13 @@ struct Tube : Shape {
 Shape *MakeBox(const double *p, int n) { return n == 3 ? new Box(p[0], p[1], p[2]) : nullptr; }
 Shape *MakeTube(const double *p, int n) { return n == 5 ? new Tube(p[0], p[1], p[2], p[3], p[4]) : nullptr; }
 
+// A tube whose rmin / phi treatment is fixed by its type rather than by its
+// parameters (TubeTypes::HollowTube, NonHollowTube: shapetypes/TubeTypes.h:84-160).
+Shape *MakeTubeTreat(const double *p, bool treatRmin, bool treatPhi) {
+  auto *t = new Tube(p[0], p[1], p[2], p[3], p[4]);
+  t->hasRmin = treatRmin;
+  t->hasPhi = treatPhi;
+  return t;
+}
+
 }  // namespace vgo

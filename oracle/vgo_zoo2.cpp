@@ -572,5 +572,4 @@ struct Trapezoid : Shape {
 
 Shape *MakeTrapezoid(const double *p, int n) { return n == 11 ? new Trapezoid(p) : nullptr; }
 
-Shape *MakePolyhedron(const double *, int) { return nullptr; }
 }  // namespace vgo

@@ -42,12 +42,26 @@ QUERIES = {
 }
 
 
-# The reference's Polyhedron dereferences out of range for points on or next
-# to the outer z-planes (FindZSegmentKernel returns nz-1 there,
-# volumes/kernel/PolyhedronImplementation.h:358-371) and the process
-# segfaults; its vectors come from tests/golden/make_golden_polyhedron.py,
-# which screens points in a child process.
-SKIP = {"Polyhedron"}
+# The reference's Polyhedron indexes its segment array out of range in
+# Contains/Inside for points inside its bounding tube but not strictly between
+# the end planes (FindZSegment gives -1 or n there;
+# volumes/kernel/PolyhedronImplementation.h:731-770, 772-830), which can crash
+# the process.  Sampled points whose local z lies within 1e-8 mm of an end
+# plane are moved 2e-8 mm beyond it (outside the bounding tube, which reaches
+# kTolerance past the planes).
+SKIP = set()
+END_BAND = 1e-8
+
+
+def screen_polyhedron_ends(unplaced, loc):
+    nz = int(unplaced.params[3])
+    z = unplaced.params[4:4 + nz]
+    lo, hi = z[0], z[-1]
+    near_lo = np.abs(loc[2] - lo) < END_BAND
+    near_hi = np.abs(loc[2] - hi) < END_BAND
+    loc[2, near_lo] = lo - 2 * END_BAND
+    loc[2, near_hi] = hi + 2 * END_BAND
+    return int(near_lo.sum() + near_hi.sum())
 
 
 def same(a, b):
@@ -79,8 +93,12 @@ def make_case(ref, ref_fast, name, unplaced, xf, seed):
         if ok[k]:
             sig = 1e-7 if k < n2 else 5e-10
             loc[:, k] = lp_tmp[:, k] + step[k] * ld[:, k] + rng.normal(0.0, sig, 3)
+    if unplaced.type == G.VG_POLYHEDRON:
+        screen_polyhedron_ends(unplaced, loc)
     p = W.local_to_master(xf, loc)
     lp = W.master_to_local(xf, p)
+    if unplaced.type == G.VG_POLYHEDRON:
+        assert screen_polyhedron_ends(unplaced, lp) == 0
     sm = np.full(N, 1e30)
     sm[::7] = rng.uniform(0.0, 50.0, len(sm[::7]))  # some finite step limits
     out = {"p": p, "d": d, "lp": lp, "ld": ld, "sm": sm, "xf": xf12,
@@ -108,8 +126,13 @@ def main():
     if ref is None:
         raise SystemExit("oracle/_ref/libvgref.so is not built (needs the reference tree)")
     ref_fast = po.reference(fast=True)
+    only = set(sys.argv[1:])  # shape names to regenerate (default: all)
     manifest = {"generator": "tests/golden/make_golden.py", "oracle": "reference Scalar backend, strict FP",
                 "N": N, "cases": {}}
+    mpath = os.path.join(OUT, "manifest.json")
+    if only and os.path.exists(mpath):
+        with open(mpath) as f:
+            manifest = json.load(f)
     rng = np.random.Generator(np.random.Philox(W.SEED_BASE + 100))
     shapes = W.config1_shapes() + W.zoo_shapes()
     seen = set()
@@ -120,6 +143,8 @@ def main():
         seen.add(name)
         for kind in ("id", "rot"):
             xf = G.Transformation3D() if kind == "id" else W.random_placement(rng)
+            if only and name not in only:
+                continue
             fn, fm = make_case(ref, ref_fast, tag, u, xf, W.SEED_BASE + 1000 + 10 * idx + (kind == "rot"))
             manifest["cases"][fn] = {"shape": name, "params": u.params, "placement": kind,
                                      "fast_math_mismatches": fm}

@@ -15,7 +15,7 @@ from vecgeom_amd import geometry as G
 from vecgeom_amd import workloads as W
 
 LIB = os.path.join(ROOT, "tests", "native", "libnavhost.so")
-DEVICE_TYPES = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11}
+DEVICE_TYPES = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12}
 QUERIES = [("contains", po.Q_CONTAINS, 0), ("inside", po.Q_INSIDE, 1), ("dist_in", po.Q_DIST_IN, 2),
            ("dist_out", po.Q_DIST_OUT, 3), ("safe_in", po.Q_SAFE_IN, 4), ("safe_out", po.Q_SAFE_OUT, 5)]
 EXTRA = [  # phi-sectioned and degenerate variants beyond the zoo
@@ -30,6 +30,14 @@ EXTRA = [  # phi-sectioned and degenerate variants beyond the zoo
     ("TorusPhi", G.UnplacedTorus(5, 10, 50, 0.3, 4.0)),
     ("TorusSolid", G.UnplacedTorus(0, 10, 40, 0, 2 * np.pi)),
     ("PolyconeOne", G.UnplacedPolycone(0.0, 2 * np.pi, rmin=[0, 4], rmax=[20, 30], z=[-30, 30])),
+    ("PolyhedronPhi", G.UnplacedPolyhedron(20.0, 250.0, 5, z=[-40, -10, 10, 40], rmin=[5, 8, 8, 5],
+                                           rmax=[25, 35, 30, 20])),
+    ("PolyhedronLargeCut", G.UnplacedPolyhedron(-30.0, 120.0, 3, z=[-30, 0, 30], rmin=[0, 5, 0], rmax=[20, 25, 20])),
+    ("PolyhedronSolid", G.UnplacedPolyhedron(0.0, 360.0, 6, z=[-20, 0, 20], rmin=[0, 0, 0], rmax=[15, 25, 15])),
+    ("PolyhedronDupZ", G.UnplacedPolyhedron(0.0, 360.0, 7, z=[-30, -10, -10, 15, 30], rmin=[5, 5, 12, 8, 6],
+                                            rmax=[20, 25, 35, 30, 22])),
+    ("PolyhedronDupZPhi", G.UnplacedPolyhedron(45.0, 300.0, 4, z=[-30, -10, -10, 15, 30], rmin=[0, 2, 12, 8, 6],
+                                               rmax=[20, 25, 35, 30, 22])),
 ]
 
 

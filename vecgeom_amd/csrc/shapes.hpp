@@ -7,3 +7,4 @@
 #include "shapes_cone.hpp"
 #include "shapes_polycone.hpp"
 #include "shapes_torus.hpp"
+#include "shapes_polyhedron.hpp"

@@ -1,0 +1,395 @@
+// Polyhedron (regular n-sided sections stacked along z): parameter block and
+// device kernels with the semantics of the scalar kernels of
+// volumes/kernel/PolyhedronImplementation.h:317-1010 over the quadrilateral
+// sets of volumes/Quadrilaterals.h:94-572 and the planes of
+// volumes/Planes.h:207-276, as UnplacedPolyhedron builds and specialises
+// them (source/UnplacedPolyhedron.cpp:27-320).
+//
+// Where the reference indexes out of range, results are defined:
+//  * Inside / Contains of a point inside the bounding tube but not strictly
+//    between the end planes (the reference indexes its segment array with -1
+//    or n): the end segment is used and kInside becomes kSurface; Contains is
+//    false.
+//  * the phi-wedge test of an end segment that is degenerate in z (no phi
+//    quadrilaterals): skipped.
+//
+// The block has a variable length (every quadrilateral is precomputed on
+// the host), so kernels receive a pointer to it in device memory:
+//   [0] nseg  [1] sides  [2] hasPhi  [3] largePhi  [4] bounding-tube z offset
+//   [5] offset of the z planes (nseg + 1)  [6] offset of the phi-section
+//   vectors (3 per side boundary)  [7] offset of the segment records
+//   [16 ..] TubeP of the bounding tube with its treatment flags fixed as the
+//   reference's HollowTube / NonHollowTube kernels fix them
+//   segment record (8): hasInner, nOuter, nPhi, nInner, offOuter, offPhi,
+//   offInner, unused
+//   quadrilateral record (32): plane normal (3) and distance, four side
+//   planes (normal 3, distance), four corners (3 each)
+#pragma once
+#include "shapes_polycone.hpp"
+
+namespace vga {
+
+struct PolyhedronP {
+  const double *blk;
+};
+
+struct PolyhedronS {
+  using P = PolyhedronP;
+  static constexpr int kTubeOff = 16;
+  static constexpr int kSegDoubles = 8;
+  static constexpr int kQuadDoubles = 32;
+
+  struct Quads {
+    const double *q;  // first record
+    int n;
+    VGA_HD const double *R(int i) const { return q + kQuadDoubles * i; }
+    VGA_HD V3 N(int i) const { return V3{R(i)[0], R(i)[1], R(i)[2]}; }
+    VGA_HD double D(int i) const { return R(i)[3]; }
+    VGA_HD V3 SN(int i, int j) const { return V3{R(i)[4 + 4 * j], R(i)[5 + 4 * j], R(i)[6 + 4 * j]}; }
+    VGA_HD double SD(int i, int j) const { return R(i)[7 + 4 * j]; }
+    VGA_HD V3 C(int i, int j) const { return V3{R(i)[20 + 3 * j], R(i)[21 + 3 * j], R(i)[22 + 3 * j]}; }
+    VGA_HD double Plane(int i, V3 p) const { return Dot(p, N(i)) + D(i); }
+  };
+  struct Seg {
+    bool hasInner;
+    Quads outer, phi, inner;
+  };
+
+  static VGA_HD int NSeg(const P &q) { return (int)q.blk[0]; }
+  static VGA_HD int Sides(const P &q) { return (int)q.blk[1]; }
+  static VGA_HD bool HasPhi(const P &q) { return q.blk[2] != 0; }
+  static VGA_HD bool LargePhi(const P &q) { return q.blk[3] != 0; }
+  static VGA_HD double Z(const P &q, int i) { return q.blk[(int)q.blk[5] + i]; }
+  static VGA_HD V3 PhiSec(const P &q, int i) {
+    const double *v = q.blk + (int)q.blk[6] + 3 * i;
+    return V3{v[0], v[1], v[2]};
+  }
+  static VGA_HD const TubeP &Tube(const P &q) { return *reinterpret_cast<const TubeP *>(q.blk + kTubeOff); }
+  static VGA_HD V3 BoundsPoint(const P &q, V3 p) { return V3{p.x, p.y, p.z - q.blk[4]}; }
+  static VGA_HD Seg GetSeg(const P &q, int k) {
+    const double *r = q.blk + (int)q.blk[7] + kSegDoubles * k;
+    Seg s;
+    s.hasInner = r[0] != 0;
+    s.outer = Quads{q.blk + (int)r[4], (int)r[1]};
+    s.phi = Quads{q.blk + (int)r[5], (int)r[2]};
+    s.inner = Quads{q.blk + (int)r[6], (int)r[3]};
+    return s;
+  }
+
+  // DistanceToLineSegmentSquared<kScalar>, GenericKernels.h:118-147
+  static VGA_HD double SegmentDistanceSquared(V3 c0, V3 c1, V3 p) {
+    const V3 line = c1 - c0;
+    const V3 proj = p - c0;
+    const double dot0 = Dot(proj, line);
+    if (dot0 <= 0) return (p - c0).Mag2();
+    const double dot1 = line.Mag2();
+    if (dot1 <= dot0) return (p - c1).Mag2();
+    return ((c0 + (dot0 / dot1) * line) - p).Mag2();
+  }
+
+  // Planes::Contains, Planes.h:207-219
+  static VGA_HD bool QContains(const Quads &s, V3 p) {
+    bool r = true;
+    for (int i = 0; i < s.n; ++i) r &= s.Plane(i, p) < 0;
+    return r;
+  }
+  // Planes::Inside, Planes.h:252-270
+  static VGA_HD int QInside(const Quads &s, V3 p) {
+    int r = kInside;
+    for (int i = 0; i < s.n; ++i) {
+      const double *R = s.R(i);
+      const double v = R[0] * p.x + R[1] * p.y + R[2] * p.z + R[3];
+      if (v > kTolerance) r = kOutside;
+      if (r == kInside && v > -kTolerance) r = kSurface;
+      if (r == kOutside) break;
+    }
+    return r;
+  }
+  static VGA_HD bool RayHits(const Quads &s, int i, V3 x) {
+    bool valid = true;
+    for (int j = 0; j < 4; ++j) {
+      valid &= Dot(x, s.SN(i, j)) + s.SD(i, j) >= 0;
+      if (!valid) break;
+    }
+    return valid;
+  }
+  // Quadrilaterals::DistanceToIn<kScalar, behind>, Quadrilaterals.h:302-350
+  template <bool Behind>
+  static VGA_HD double QDistanceToIn(const Quads &s, V3 p, V3 d) {
+    double best = kInfinity;
+    for (int i = 0; i < s.n; ++i) {
+      const V3 nn = s.N(i);
+      double distance = Dot(p, nn) + s.D(i);
+      if (!((Behind ? -distance : distance) >= 0)) continue;
+      const double proj = Dot(d, nn);
+      if (!((Behind ? proj : -proj) >= 0)) continue;
+      distance /= -proj;
+      const V3 x = p + d * distance;
+      if (RayHits(s, i, x)) best = distance;
+      if (best < kInfinity) break;
+    }
+    return best;
+  }
+  // Quadrilaterals::DistanceToOut<kScalar>, Quadrilaterals.h:435-490
+  static VGA_HD double QDistanceToOut(const Quads &s, V3 p, V3 d, double zMin, double zMax) {
+    double best = kInfinity;
+    for (int i = 0; i < s.n; ++i) {
+      const V3 nn = s.N(i);
+      double t = Dot(p, nn) + s.D(i);
+      if (!(t < 0)) continue;
+      const double proj = Dot(d, nn);
+      if (!(proj >= 0)) continue;
+      t /= -proj;
+      if (!(t < best)) continue;
+      bool valid;
+      if (zMin == zMax) {
+        valid = RayHits(s, i, p + t * d);
+      } else {
+        const double zp = p.z + t * d.z;
+        valid = zp >= zMin && zp < zMax;
+      }
+      if (valid) best = t;
+    }
+    return best;
+  }
+  // Quadrilaterals::ScalarDistanceSquared, Quadrilaterals.h:500-568
+  static VGA_HD double QDistanceSquared(const Quads &s, int i, V3 p) {
+    const V3 nn = s.N(i);
+    const double distance = Dot(p, nn) + s.D(i);
+    const V3 x = p + distance * nn;
+    bool w[4];
+    for (int j = 0; j < 4; ++j) {
+      const V3 sn = s.SN(i, j);
+      w[j] = x.x * sn.x + x.y * sn.y + x.z * sn.z + s.SD(i, j) >= 0;
+    }
+    if (w[0] && w[1] && w[2] && w[3]) return distance * distance;
+    const int a = !w[0] ? 0 : (!w[2] ? 2 : (!w[1] ? 1 : 3));
+    return SegmentDistanceSquared(s.C(i, a), s.C(i, (a + 1) & 3), p);
+  }
+
+  // FindZSegmentKernel<kScalar>, PolyhedronImplementation.h:317-331
+  static VGA_HD int FindZ(const P &q, double pz) {
+    const int nz = NSeg(q) + 1;
+    int index = -1;
+    for (int k = 0; k < nz; ++k) {
+      if (!(pz > Z(q, k))) break;
+      ++index;
+    }
+    return index;
+  }
+  static VGA_HD int ClampZ(const P &q, int k) {
+    const int n = NSeg(q);
+    return k < 0 ? 0 : (k >= n ? n - 1 : k);
+  }
+  // FindPhiSegment, :357-385
+  static VGA_HD int FindPhi(const P &q, V3 p) {
+    int index = -1;
+    V3 s = PhiSec(q, 0);
+    double first = p.x * s.x + p.y * s.y + p.z * s.z;
+    const int n = Sides(q) + 1;
+    for (int i = 1; i < n; ++i) {
+      s = PhiSec(q, i);
+      const double second = p.x * s.x + p.y * s.y + p.z * s.z;
+      if (first >= 0 && second < 0) index = i - 1;
+      if (index >= 0) break;
+      first = second;
+    }
+    return index;
+  }
+  // InPhiCutoutWedge, :679-700
+  static VGA_HD bool InPhiWedge(const P &q, const Seg &s, V3 p) {
+    if (s.phi.n != 2) return false;
+    const bool first = s.phi.Plane(0, p) >= 0;
+    const bool second = s.phi.Plane(1, p) >= 0;
+    return LargePhi(q) ? (first || second) : (first && second);
+  }
+  // ScalarSegmentContainsKernel, :702-726
+  static VGA_HD bool SegmentContains(const P &q, V3 p, int k) {
+    const Seg s = GetSeg(q, k);
+    if (!QContains(s.outer, p)) return false;
+    if (s.hasInner && QContains(s.inner, p)) return false;
+    if (HasPhi(q)) {
+      bool cin, cout;
+      WedgeS::Kernel<false>(Tube(q).w, p, cin, cout);
+      if (cout) return false;
+    }
+    return true;
+  }
+
+  // ScalarContainsKernel, :728-770
+  static VGA_HD bool Contains(const P &q, V3 p) {
+    if (!TubeS::Contains(Tube(q), BoundsPoint(q, p))) return false;
+    const int k = FindZ(q, p.z);
+    if (!(k >= 0) || k >= NSeg(q)) return false;
+    const Seg s = GetSeg(q, k);
+    if (!QContains(s.outer, p)) return false;
+    if (s.hasInner && QContains(s.inner, p)) return false;
+    return true;
+  }
+
+  // ScalarInsideKernel, :772-830
+  static VGA_HD int Inside(const P &q, V3 p) {
+    if (!TubeS::Contains(Tube(q), BoundsPoint(q, p))) return kOutside;
+    const int k0 = FindZ(q, p.z);
+    const bool endBand = k0 < 0 || k0 >= NSeg(q);
+    const Seg s = GetSeg(q, ClampZ(q, k0));
+    int r = QInside(s.outer, p);
+    if (r == kInside) {
+      const int ii = s.hasInner ? QInside(s.inner, p) : kOutside;
+      if (ii == kInside)
+        r = kOutside;
+      else if (ii == kSurface)
+        r = kSurface;
+      else if (HasPhi(q) && InPhiWedge(q, s, p))
+        r = kOutside;
+    }
+    if (endBand && r == kInside) r = kSurface;
+    return r;
+  }
+
+  // DistanceToInZSegment, :387-428
+  static VGA_HD double DistanceToInSegment(const P &q, int k, V3 p, V3 d) {
+    const Seg s = GetSeg(q, k);
+    double distance = QDistanceToIn<false>(s.outer, p, d);
+    if (distance < kInfinity) return distance;
+    if (HasPhi(q)) distance = QDistanceToIn<false>(s.phi, p, d);
+    if (distance < kInfinity) return distance;
+    if (s.hasInner) distance = QDistanceToIn<true>(s.inner, p, d);
+    return distance;
+  }
+  // DistanceToOutZSegment, :430-476
+  static VGA_HD double DistanceToOutSegment(const P &q, int k, V3 p, V3 d) {
+    const Seg s = GetSeg(q, k);
+    bool done = false;
+    double distance = kInfinity;
+    if (s.hasInner) {
+      distance = QDistanceToIn<false>(s.inner, p, d);
+      done = distance < kInfinity;
+      if (done) return distance;
+    }
+    if (HasPhi(q)) {
+      const double dphi = QDistanceToIn<true>(s.phi, p, d);
+      if (!done && distance >= 0.) distance = dphi;
+      done = distance >= 0. && distance < kInfinity;
+      if (done) return distance;
+    }
+    const double dout = QDistanceToOut(s.outer, p, d, Z(q, k), Z(q, k + 1));
+    if (!done) distance = dout;
+    return distance;
+  }
+  // ScalarSafetyToZSegmentSquared, :478-521
+  static VGA_HD double SafetySegmentSquared(const P &q, int k, int phiIndex, V3 p) {
+    const Seg s = GetSeg(q, k);
+    if (HasPhi(q) && s.phi.n == 2 && InPhiWedge(q, s, p))
+      return Min(QDistanceSquared(s.phi, 0, p), QDistanceSquared(s.phi, 1, p));
+    if (phiIndex < 0) return kInfinity;
+    double so = kInfinity;
+    if (s.outer.n > 0) so = QDistanceSquared(s.outer, phiIndex, p);
+    double si = kInfinity;
+    if (s.hasInner && s.inner.n > 0) si = QDistanceSquared(s.inner, phiIndex, p);
+    return Min(si, so);
+  }
+  // The endcap tests of ScalarDistanceToEndcaps / ScalarSafetyToEndcapsSquared.
+  static VGA_HD bool OnEndcap(const P &q, const Seg &s, V3 x) {
+    if (!QContains(s.outer, x)) return false;
+    if (s.hasInner && QContains(s.inner, x)) return false;
+    if (HasPhi(q) && InPhiWedge(q, s, x)) return false;
+    return true;
+  }
+  // ScalarDistanceToEndcaps<pointInside>, :523-602
+  template <bool PointInside>
+  static VGA_HD void DistanceToEndcaps(const P &q, V3 p, V3 d, double &distance) {
+    const bool useFirst = PointInside ? (d.z < 0) : !(d.z < 0);
+    const int n = NSeg(q);
+    const double zPlane = useFirst ? Z(q, 0) : Z(q, n);
+    const double t = (zPlane - p.z) / d.z;
+    if (t < 0 || t >= distance) return;
+    if (OnEndcap(q, GetSeg(q, useFirst ? 0 : n - 1), p + t * d)) distance = t;
+  }
+  // ScalarSafetyToEndcapsSquared, :604-645
+  static VGA_HD void SafetyToEndcapsSquared(const P &q, V3 p, double &distSq) {
+    const int n = NSeg(q);
+    const double firstD = Z(q, 0) - p.z;
+    const double lastD = Z(q, n) - p.z;
+    const bool isFirst = Abs(firstD) < Abs(lastD);
+    const double t = isFirst ? firstD : lastD;
+    const double tSq = t * t;
+    if (tSq >= distSq) return;
+    if (OnEndcap(q, GetSeg(q, isFirst ? 0 : n - 1), V3{p.x, p.y, p.z + t})) distSq = tSq;
+  }
+
+  // ScalarDistanceToInKernel, :832-913 (local point and direction)
+  static VGA_HD double DistanceToIn(const P &q, V3 p, V3 d, double stepMax) {
+    const V3 bp = BoundsPoint(q, p);
+    const bool inBounds = TubeS::Contains(Tube(q), bp);
+    if (!inBounds && TubeS::DistanceToIn(Tube(q), bp, d, stepMax) == kInfinity) return kInfinity;
+    const int n = NSeg(q);
+    int k = ClampZ(q, FindZ(q, p.z));
+    if (inBounds && SegmentContains(q, p, k)) return -1.;
+    double distance = kInfinity;
+    if (d.z >= 0) {
+      for (; k < n; ++k) {
+        distance = DistanceToInSegment(q, k, p, d);
+        if (distance >= 0 && distance < kInfinity) break;
+      }
+    } else {
+      for (; k >= 0; --k) {
+        distance = DistanceToInSegment(q, k, p, d);
+        if (distance >= 0 && distance < kInfinity) break;
+      }
+    }
+    DistanceToEndcaps<false>(q, p, d, distance);
+    return distance < stepMax ? distance : stepMax;
+  }
+
+  // ScalarDistanceToOutKernel, :946-1004
+  static VGA_HD double DistanceToOut(const P &q, V3 p, V3 d, double stepMax) {
+    const int n = NSeg(q);
+    int k = ClampZ(q, FindZ(q, p.z));
+    double distance = kInfinity;
+    if (d.z >= 0) {
+      for (; k < n; ++k) {
+        distance = DistanceToOutSegment(q, k, p, d);
+        if (distance >= 0 && distance < kInfinity) break;
+        if (Z(q, k) - p.z > distance) break;
+      }
+    } else {
+      for (; k >= 0; --k) {
+        distance = DistanceToOutSegment(q, k, p, d);
+        if (distance >= 0 && distance < kInfinity) break;
+        if (p.z - Z(q, k) > distance) break;
+      }
+    }
+    DistanceToEndcaps<true>(q, p, d, distance);
+    return distance < stepMax ? distance : stepMax;
+  }
+
+  // ScalarSafetyKernel, :915-944 (both SafetyToIn and SafetyToOut)
+  static VGA_HD double Safety(const P &q, V3 p) {
+    const int n = NSeg(q);
+    double safety = kInfinity;
+    const int k = ClampZ(q, FindZ(q, p.z));
+    const int phiIndex = FindPhi(q, p);
+    for (int z = k; z < n;) {
+      safety = Min(safety, SafetySegmentSquared(q, z, phiIndex, p));
+      ++z;
+      if (Z(q, z) - p.z > safety) break;
+    }
+    for (int z = k - 1; z >= 0; --z) {
+      safety = Min(safety, SafetySegmentSquared(q, z, phiIndex, p));
+      if (p.z - Z(q, z) > safety) break;
+    }
+    SafetyToEndcapsSquared(q, p, safety);
+    return Sqrt(safety);
+  }
+  static VGA_HD double SafetyToIn(const P &q, V3 p) { return Safety(q, p); }
+  static VGA_HD double SafetyToOut(const P &q, V3 p) { return Safety(q, p); }
+};
+
+template <>
+struct ParamView<PolyhedronS> {
+  static constexpr bool kIndirect = true;
+  static VGA_HD PolyhedronP Get(const double *b) { return PolyhedronP{b}; }
+};
+
+}  // namespace vga

@@ -136,7 +136,8 @@ __device__ __forceinline__ void RunShape(const TaskDev t, const double *params, 
 constexpr int kClasses = 3;
 
 constexpr int ClassOf(int type) {
-  return (type == VG_CONE || type == VG_SPHERE || type == VG_POLYCONE) ? 1 : (type == VG_TORUS ? 2 : 0);
+  return (type == VG_CONE || type == VG_SPHERE || type == VG_POLYCONE || type == VG_POLYHEDRON) ? 1
+                                                                                                 : (type == VG_TORUS ? 2 : 0);
 }
 
 template <int IT, int CLS>
@@ -156,6 +157,7 @@ __device__ __forceinline__ void RunTask(const TaskDev t, const double *params, u
       case VG_SPHERE: RunShape<SphereS, IT>(t, params, i0); break;
       case VG_CONE: RunShape<ConeS, IT>(t, params, i0); break;
       case VG_POLYCONE: RunShape<PolyconeS, IT>(t, params, i0); break;
+      case VG_POLYHEDRON: RunShape<PolyhedronS, IT>(t, params, i0); break;
     }
   } else {
     if (t.type == VG_TORUS) RunShape<TorusS, IT>(t, params, i0);

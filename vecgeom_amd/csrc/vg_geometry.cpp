@@ -34,7 +34,7 @@ int CheckHip(hipError_t e, const char *what) {
 bool ShapeHasKernels(int type) {
   switch (type) {
     case VG_BOX: case VG_TUBE: case VG_ORB: case VG_TRD: case VG_PARALLELEPIPED: case VG_PARABOLOID:
-    case VG_TRAPEZOID: case VG_SPHERE: case VG_CONE: case VG_POLYCONE: case VG_TORUS:
+    case VG_TRAPEZOID: case VG_SPHERE: case VG_CONE: case VG_POLYCONE: case VG_TORUS: case VG_POLYHEDRON:
       return true;
   }
   return false;

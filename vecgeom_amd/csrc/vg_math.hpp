@@ -44,10 +44,12 @@ template <bool tol> VGA_HD double MinusTolSq(double x, double xsq) { return tol 
 struct V3 {
   double x, y, z;
   VGA_HD double operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+  VGA_HD double Mag2() const { return x * x + y * y + z * z; }
 };
 VGA_HD V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 VGA_HD V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 VGA_HD V3 operator*(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+VGA_HD V3 operator*(V3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
 VGA_HD double Dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
 // Placement transform.  Kinds select the reference's specialised

@@ -102,6 +102,10 @@ struct NavDev {
       using S = TorusS;                        \
       BODY;                                    \
     }                                          \
+    case VG_POLYHEDRON: {                      \
+      using S = PolyhedronS;                   \
+      BODY;                                    \
+    }                                          \
   }
 
 #define VGA_P(blk) (ParamView<S>::Get(blk))

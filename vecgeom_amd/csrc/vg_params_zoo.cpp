@@ -63,6 +63,185 @@ struct TrapBuild {
   }
 };
 
+
+// ------------------------------------------------------------ Polyhedron
+// UnplacedPolyhedron's construction (source/UnplacedPolyhedron.cpp:27-233)
+// with the quadrilateral set-up of source/Quadrilaterals.cpp:49-140 and
+// Planes::Set (source/Planes.cpp:38-47), written into the block layout
+// PolyhedronS reads.
+V3 PhCross(V3 l, V3 r) { return {l.y * r.z - l.z * r.y, l.z * r.x - l.x * r.z, l.x * r.y - l.y * r.x}; }
+double PhMag(V3 v) { return std::sqrt(v.Mag2()); }
+V3 PhUnit(V3 v) { return v * (1. / PhMag(v)); }
+V3 PhFixZeroes(V3 v) {
+  double c[3] = {v.x, v.y, v.z};
+  for (double &x : c)
+    if (std::fabs(x) < kTolerance) x = 0.;
+  return V3{c[0], c[1], c[2]};
+}
+// Vector3D equality is within kTolerance per component (base/Vector3D.h:751-757).
+bool PhSame(V3 a, V3 b) {
+  return std::fabs(a.x - b.x) < kTolerance && std::fabs(a.y - b.y) < kTolerance && std::fabs(a.z - b.z) < kTolerance;
+}
+double PhNormalizeAngle(double a) { return a + 2 * kPi * ((a < 0) - int(a / (2 * kPi))); }
+
+// One quadrilateral record: plane, four side planes, four corners.
+void PhQuad(double *r, V3 c0, V3 c1, V3 c2, V3 c3) {
+  const V3 cs[4] = {c0, c1, c2, c3};
+  // Three corners for the plane, skipping a corner equal to the previous one.
+  V3 ch[4];
+  ch[0] = cs[0];
+  int assigned = 1, tested = 1;
+  while (tested <= 4) {
+    do {
+      ch[assigned] = cs[tested];
+      tested++;
+    } while (PhSame(ch[assigned], ch[assigned - 1]) && tested < 4);
+    assigned++;
+    if (assigned == 3) break;
+  }
+  const double a = ch[0].y * (ch[1].z - ch[2].z) + ch[1].y * (ch[2].z - ch[0].z) + ch[2].y * (ch[0].z - ch[1].z);
+  const double b = ch[0].z * (ch[1].x - ch[2].x) + ch[1].z * (ch[2].x - ch[0].x) + ch[2].z * (ch[0].x - ch[1].x);
+  const double c = ch[0].x * (ch[1].y - ch[2].y) + ch[1].x * (ch[2].y - ch[0].y) + ch[2].x * (ch[0].y - ch[1].y);
+  double d = -ch[0].x * (ch[1].y * ch[2].z - ch[2].y * ch[1].z) - ch[1].x * (ch[2].y * ch[0].z - ch[0].y * ch[2].z) -
+             ch[2].x * (ch[0].y * ch[1].z - ch[1].y * ch[0].z);
+  V3 nrm{a, b, c};
+  const double inv = 1. / PhMag(nrm);
+  nrm = nrm * inv;
+  d *= inv;
+  nrm = PhFixZeroes(nrm);
+  r[0] = nrm.x; r[1] = nrm.y; r[2] = nrm.z; r[3] = d;
+  for (int j = 0; j < 4; ++j) {
+    const V3 s0 = cs[j], s1 = cs[(j + 1) & 3];
+    double *sr = r + 4 + 4 * j;
+    if (!PhSame(s1, s0)) {
+      const V3 f = PhFixZeroes(PhUnit(PhCross(nrm, s1 - s0)));
+      const double il = 1. / PhMag(f);
+      sr[0] = il * f.x; sr[1] = il * f.y; sr[2] = il * f.z;
+      sr[3] = il * -Dot(f, s0);
+    } else {
+      sr[0] = sr[1] = sr[2] = sr[3] = 0.;
+    }
+    r[20 + 3 * j] = cs[j].x; r[21 + 3 * j] = cs[j].y; r[22 + 3 * j] = cs[j].z;
+  }
+}
+void PhFlip(double *r) {
+  for (int k = 0; k < 4; ++k) r[k] = -r[k];
+}
+
+std::vector<double> BuildPolyhedron(const double *c, int n, std::string *err) {
+  if (n < 4) return Fail(err, "polyhedron needs phiStart phiDelta sides nz z[nz] rmin[nz] rmax[nz]");
+  const int sides = (int)c[2], nz = (int)c[3];
+  if (sides < 1 || nz < 2 || n != 4 + 3 * nz) return Fail(err, "polyhedron needs sides >= 1, nz >= 2, 4 + 3 nz values");
+  if (!(c[1] > 0)) return Fail(err, "polyhedron needs phiDelta > 0");
+  const double *z = c + 4, *rmin = c + 4 + nz, *rmax = c + 4 + 2 * nz;
+  const int nseg = nz - 1;
+  for (int i = 0; i < nseg; ++i)
+    if (!(z[i] <= z[i + 1])) return Fail(err, "polyhedron z planes must not decrease");
+  for (int i = 0; i < nz; ++i)
+    if (!(rmin[i] >= 0 && rmax[i] > 0)) return Fail(err, "polyhedron needs rmin >= 0 and rmax > 0");
+  const bool hasPhi = c[1] < 360, largePhi = c[1] < 180;
+  struct SegCount {
+    bool inner;
+    int nOuter, nPhi, nInner;
+  };
+  std::vector<SegCount> sc(nseg);
+  bool hasInnerRadii = false;
+  for (int i = 0; i < nseg; ++i) {
+    sc[i].inner = rmin[i] > 0 || rmin[i + 1] > 0;
+    sc[i].nOuter = (z[i] == z[i + 1] && rmax[i] == rmax[i + 1]) ? 0 : sides;
+    sc[i].nPhi = hasPhi ? (z[i] == z[i + 1] ? 0 : 2) : 0;
+    sc[i].nInner = sc[i].inner ? ((z[i] == z[i + 1] && rmin[i] == rmin[i + 1]) ? 0 : sides) : 0;
+    hasInnerRadii |= sc[i].inner;
+  }
+  const double phiStart = PhNormalizeAngle(kDegToRad * c[0]);
+  double phiDelta = c[1] * kDegToRad;
+  if (phiDelta > 2 * kPi) phiDelta = 2 * kPi;
+  const double sidePhi = phiDelta / sides;
+  std::vector<double> vphi(sides + 1);
+  std::vector<V3> phiSec(sides + 1);
+  for (int i = 0; i <= sides; ++i) {
+    vphi[i] = PhNormalizeAngle(phiStart + i * sidePhi);
+    const V3 cv = PhFixZeroes(PhUnit(V3{1. * std::cos(vphi[i]), 1. * std::sin(vphi[i]), 0}));
+    phiSec[i] = PhCross(PhUnit(cv), V3{0, 0, -1});
+  }
+  if (!hasPhi) vphi[sides] = vphi[0];
+  const double cosHalf = std::cos(0.5 * sidePhi);
+  double innerRadius = kInfinity, outerRadius = -kInfinity;
+  for (int i = 0; i < nz; ++i) {
+    if (rmin[i] < innerRadius) innerRadius = rmin[i];
+    if (rmax[i] > outerRadius) outerRadius = rmax[i];
+  }
+  outerRadius /= cosHalf;
+  const double tubeZ = z[nz - 1] - z[0] + 2. * kTolerance;
+  TubeP tube = MakeTube(innerRadius - kTolerance, outerRadius + kTolerance, 0.5 * tubeZ, !hasPhi ? 0 : phiStart,
+                        !hasPhi ? 2 * kPi : phiDelta);
+  // The reference bounds the polyhedron with a HollowTube or NonHollowTube
+  // kernel: rmin treated iff there are inner radii, phi never.
+  tube.hasRmin = hasInnerRadii ? 1. : 0.;
+  tube.hasPhi = 0.;
+
+  // Layout.
+  const int tubeDoubles = (int)((sizeof(TubeP) + sizeof(double) - 1) / sizeof(double));
+  const int zOff = PolyhedronS::kTubeOff + ((tubeDoubles + 1) & ~1);
+  const int phiOff = zOff + ((nz + 1) & ~1);
+  const int segOff = phiOff + ((3 * (sides + 1) + 1) & ~1);
+  int quadOff = segOff + PolyhedronS::kSegDoubles * nseg;
+  int nquads = 0;
+  for (auto &s : sc) nquads += s.nOuter + s.nPhi + s.nInner;
+  std::vector<double> blk(quadOff + (size_t)PolyhedronS::kQuadDoubles * nquads, 0.);
+  blk[0] = nseg; blk[1] = sides; blk[2] = hasPhi; blk[3] = largePhi; blk[4] = z[0] + 0.5 * tubeZ;
+  blk[5] = zOff; blk[6] = phiOff; blk[7] = segOff;
+  std::memcpy(blk.data() + PolyhedronS::kTubeOff, &tube, sizeof(TubeP));
+  for (int i = 0; i < nz; ++i) blk[zOff + i] = z[i];
+  for (int i = 0; i <= sides; ++i) {
+    blk[phiOff + 3 * i] = phiSec[i].x; blk[phiOff + 3 * i + 1] = phiSec[i].y; blk[phiOff + 3 * i + 2] = phiSec[i].z;
+  }
+  // Corners: rMin / rMax are distances to the sides, corners lie at r / cos(half side angle).
+  auto vi = [&](int plane, int corner) { return plane * (sides + 1) + corner; };
+  std::vector<V3> ov(nz * (sides + 1)), iv(nz * (sides + 1));
+  for (int i = 0; i < nz; ++i) {
+    for (int j = 0, jm = sides + (hasPhi ? 1 : 0); j < jm; ++j) {
+      const double ro = rmax[i] / cosHalf, ri = rmin[i] / cosHalf;
+      ov[vi(i, j)] = PhFixZeroes(V3{ro * std::cos(vphi[j]), ro * std::sin(vphi[j]), z[i]});
+      iv[vi(i, j)] = PhFixZeroes(V3{ri * std::cos(vphi[j]), ri * std::sin(vphi[j]), z[i]});
+    }
+    if (!hasPhi) {
+      ov[vi(i, sides)] = ov[vi(i, 0)];
+      iv[vi(i, sides)] = iv[vi(i, 0)];
+    }
+  }
+  // Normals of the side faces point away from the z axis.
+  auto wrong = [](const double *r, V3 corner) { return r[0] * corner.x + r[1] * corner.y < 0; };
+  for (int ip = 0; ip < nseg; ++ip) {
+    double *rec = blk.data() + segOff + PolyhedronS::kSegDoubles * ip;
+    rec[0] = sc[ip].inner; rec[1] = sc[ip].nOuter; rec[2] = sc[ip].nPhi; rec[3] = sc[ip].nInner;
+    rec[4] = quadOff;
+    for (int k = 0; k < sc[ip].nOuter; ++k) {
+      double *q = blk.data() + quadOff + PolyhedronS::kQuadDoubles * k;
+      PhQuad(q, ov[vi(ip, k)], ov[vi(ip, k + 1)], ov[vi(ip + 1, k + 1)], ov[vi(ip + 1, k)]);
+      if (wrong(q, ov[vi(ip, k)])) PhFlip(q);
+    }
+    quadOff += PolyhedronS::kQuadDoubles * sc[ip].nOuter;
+    rec[5] = quadOff;
+    if (sc[ip].nPhi == 2) {
+      double *q0 = blk.data() + quadOff, *q1 = q0 + PolyhedronS::kQuadDoubles;
+      PhQuad(q0, iv[vi(ip, 0)], iv[vi(ip + 1, 0)], ov[vi(ip + 1, 0)], ov[vi(ip, 0)]);
+      if (q0[0] * phiSec[0].x + q0[1] * phiSec[0].y + q0[2] * phiSec[0].z > 0) PhFlip(q0);
+      PhQuad(q1, ov[vi(ip, sides)], ov[vi(ip + 1, sides)], iv[vi(ip + 1, sides)], iv[vi(ip, sides)]);
+      if (q1[0] * phiSec[sides].x + q1[1] * phiSec[sides].y + q1[2] * phiSec[sides].z < 0) PhFlip(q1);
+    }
+    quadOff += PolyhedronS::kQuadDoubles * sc[ip].nPhi;
+    rec[6] = quadOff;
+    for (int k = 0; k < sc[ip].nInner; ++k) {
+      double *q = blk.data() + quadOff + PolyhedronS::kQuadDoubles * k;
+      PhQuad(q, iv[vi(ip, k)], iv[vi(ip, k + 1)], iv[vi(ip + 1, k + 1)], iv[vi(ip + 1, k)]);
+      if (wrong(q, iv[vi(ip, k)])) PhFlip(q);
+    }
+    quadOff += PolyhedronS::kQuadDoubles * sc[ip].nInner;
+  }
+  return blk;
+}
+
 }  // namespace
 
 // UnplacedCone constructor (volumes/UnplacedCone.h:131-197): the cached
@@ -290,6 +469,8 @@ std::vector<double> BuildZooParams(int type, const double *c, int n, std::string
       }
       return blk;
     }
+    case VG_POLYHEDRON:
+      return BuildPolyhedron(c, n, err);
     case VG_TRAPEZOID: {
       if (n != 11 || !(c[0] > 0 && c[3] > 0 && c[4] > 0 && c[5] > 0 && c[7] > 0 && c[8] > 0 && c[9] > 0))
         return Fail(err, "trapezoid needs 11 parameters with positive lengths");
@@ -374,6 +555,16 @@ bool ZooExtent(int type, const double *b, double lo[3], double hi[3]) {
       }
       lo[0] = -maxR; lo[1] = -maxR; lo[2] = PolyconeS::Z(q, 0);
       hi[0] = maxR; hi[1] = maxR; hi[2] = PolyconeS::Z(q, ns);
+      return true;
+    }
+    case VG_POLYHEDRON: {
+      // UnplacedPolyhedron::Extent: the bounding tube's, shifted by its z
+      // offset (source/UnplacedPolyhedron.cpp:324-330).
+      const PolyhedronP q{b};
+      const TubeP &t = PolyhedronS::Tube(q);
+      if (!ShapeExtent(VG_TUBE, reinterpret_cast<const double *>(&t), lo, hi)) return false;
+      lo[2] += b[4];
+      hi[2] += b[4];
       return true;
     }
     case VG_SPHERE: {

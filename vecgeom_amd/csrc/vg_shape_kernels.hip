@@ -313,6 +313,7 @@ int LaunchQuery(const vg_geometry *g, int32_t placed, int q, const QArgs &a, hip
     case VG_SPHERE: return LaunchShape<SphereS>(g, placed, block, pr, q, a, s);
     case VG_CONE: return LaunchShape<ConeS>(g, placed, block, pr, q, a, s);
     case VG_POLYCONE: return LaunchShape<PolyconeS>(g, placed, block, pr, q, a, s);
+    case VG_POLYHEDRON: return LaunchShape<PolyhedronS>(g, placed, block, pr, q, a, s);
     case VG_TORUS: return LaunchShape<TorusS>(g, placed, block, pr, q, a, s);
   }
   return SetError(VG_ERR_UNSUPPORTED, "no device kernels for shape type " + std::to_string(pr.type));
