@@ -389,7 +389,7 @@ def serve_bench(args, dev):
     sat = {f: run(f, zero) for f in forms}
     base_rate = nb / sat["per_basket"]["wall_s"]
     rates = {}
-    for frac in (0.5, 0.9):
+    for frac in (0.25, 0.5, 0.9):
         rate = frac * base_rate
         gaps = rng.exponential(1.0 / rate, nb)
         arrival = np.cumsum(gaps) - gaps[0]

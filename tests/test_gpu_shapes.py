@@ -12,6 +12,7 @@ import pytest
 import torch
 
 import pyoracle as po
+from test_shapes_hostcheck import EXTRA
 from conftest import GOLDEN, BEYOND_BAR_FRACTION, agree, assert_agree, close, golden_cases, load_golden, same
 from vecgeom_amd import capi
 from vecgeom_amd import geometry as G
@@ -150,6 +151,7 @@ def test_gpu_minimize_and_next_node(gpu, oracle):
     cur_np = np.full(n, 1e30)
     nxt_np = np.full(n, -1, np.int32)
     saf_np = np.full(n, 1e30)
+    saf_src = np.full(n, -1)
     for k, ((name, u), xf, pid) in enumerate(zip(shapes, xfs, ids)):
         Q.distance_to_in_minimize(geom, pid, k, p, d, cur, nxt)
         Q.safety_to_in_minimize(geom, pid, p, saf)
@@ -159,11 +161,20 @@ def test_gpu_minimize_and_next_node(gpu, oracle):
         cur_np = np.where(upd, r, cur_np)
         nxt_np = np.where(upd, k, nxt_np)
         s = o.query(po.Q_SAFE_IN, p_np)
+        saf_src = np.where(s < saf_np, k, saf_src)
         saf_np = np.where(s < saf_np, s, saf_np)
     # Lanes whose minimum is a Torus distance inherit its allowance.
     ok = close(host(cur), cur_np) & (host(nxt) == nxt_np)
     assert (~ok).sum() <= BEYOND_BAR_FRACTION["Torus"] * n
-    assert same(host(saf), saf_np).all()
+    # Safeties: bit-exact, except lanes whose minimum comes from a shape whose
+    # safety goes through libm transcendentals (held to the FP64 bar).
+    got = host(saf)
+    for k, (name, _) in enumerate(shapes):
+        lanes = saf_src == k
+        ok = agree(name, "safe_in", got[lanes], saf_np[lanes])
+        assert ok.all(), f"{name} safe_in minimum: {int((~ok).sum())} lanes differ"
+    lanes = saf_src < 0
+    assert same(got[lanes], saf_np[lanes]).all()
     # DistanceToOut with next-node output
     name, u = shapes[0]
     sm_np = rng.uniform(0, 40, n)
@@ -221,12 +232,35 @@ def test_gpu_plain_variant_matches_tuned(gpu):
             capi.call("vg_set_kernel_variant", 1)
             p, d = dev_soa(g["p"], gpu), dev_soa(g["d"], gpu)
             lp, ld = dev_soa(g["lp"], gpu), dev_soa(g["ld"], gpu)
+            sm = torch.from_numpy(g["sm"]).to(gpu)
             plain = {
                 "contains": host(Q.contains(geom, pid, p)), "inside": host(Q.inside(geom, pid, p)),
-                "dist_in": host(Q.distance_to_in(geom, pid, p, d)), "dist_out": host(Q.distance_to_out(geom, pid, lp, ld)),
+                "dist_in": host(Q.distance_to_in(geom, pid, p, d, sm)),
+                "dist_out": host(Q.distance_to_out(geom, pid, lp, ld, sm)),
                 "safe_in": host(Q.safety_to_in(geom, pid, p)), "safe_out": host(Q.safety_to_out(geom, pid, lp)),
             }
             for qn, v in plain.items():
                 assert same(v, tuned[qn]).all(), f"{name} {qn}: plain form differs from tuned kernels"
     finally:
         capi.call("vg_set_kernel_variant", 0)
+
+
+@pytest.mark.parametrize("name", [n for n, _ in EXTRA])
+def test_gpu_extra_variants_vs_oracle(gpu, oracle, name):
+    """Phi-sectioned and degenerate variants (Cone, Tube, Polycone, Torus,
+    Polyhedron -- one with a parameter block beyond the LDS stage, read from
+    global memory), random placement, all six queries against the oracle."""
+    u = dict(EXTRA)[name]
+    xf = W.random_placement(np.random.Generator(np.random.Philox(17)))
+    geom, ids, _ = G.single_shape_geometry([u], [xf])
+    n = 20000
+    inp = W.shape_inputs(geom.placed_extent(ids[0]), xf, n, seed=606)
+    g = {"p": inp["p"], "d": inp["d"], "lp": inp["lp"], "ld": inp["ld"], "sm": np.full(n, 1e30)}
+    got = run_all(geom, ids[0], g, gpu)
+    o = oracle.placed(u.type, u.params, xf.array())
+    family = "Torus" if u.type == G.VG_TORUS else ("Sphere" if u.type == G.VG_SPHERE else name)
+    for qn, q in (("contains", po.Q_CONTAINS), ("inside", po.Q_INSIDE), ("safe_in", po.Q_SAFE_IN)):
+        assert_agree(family, qn, got[qn], o.query(q, g["p"]), name)
+    assert_agree(family, "dist_in", got["dist_in"], o.query(po.Q_DIST_IN, g["p"], g["d"], g["sm"]), name)
+    assert_agree(family, "dist_out", got["dist_out"], o.query(po.Q_DIST_OUT, g["lp"], g["ld"], g["sm"]), name)
+    assert_agree(family, "safe_out", got["safe_out"], o.query(po.Q_SAFE_OUT, g["lp"]), name)

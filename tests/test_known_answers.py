@@ -1,4 +1,4 @@
-"""Known answers of the reference's unit tests (TestBox.cpp, TestTube.cpp),
+"""Known answers of the reference's unit tests (TestBox.cpp, TestTube.cpp, TestPolyhedra.cpp),
 transcribed as data in tests/golden/known_answers.json, checked on the
 oracle here and on the GPU in test_gpu_shapes.py."""
 import json
@@ -37,6 +37,8 @@ def check(row, got):
         return got >= 1e30 or np.isinf(got)
     if row["cmp"] == "le0":
         return got <= 0.0
+    if row["cmp"] == "abs":
+        return abs(float(got) - float(e)) < row["tol"]
     return approx_equal(float(got), float(e))
 
 

@@ -38,6 +38,10 @@ EXTRA = [  # phi-sectioned and degenerate variants beyond the zoo
                                             rmax=[20, 25, 35, 30, 22])),
     ("PolyhedronDupZPhi", G.UnplacedPolyhedron(45.0, 300.0, 4, z=[-30, -10, -10, 15, 30], rmin=[0, 2, 12, 8, 6],
                                                rmax=[20, 25, 35, 30, 22])),
+    # 12 sides x 12 segments: a parameter block larger than one workgroup's LDS stage
+    ("PolyhedronMany", G.UnplacedPolyhedron(10.0, 350.0, 12, z=list(np.linspace(-60, 60, 13)),
+                                            rmin=[5, 6, 8, 4, 9, 12, 3, 7, 8, 10, 6, 5, 4],
+                                            rmax=[20, 25, 30, 22, 35, 40, 18, 27, 33, 38, 26, 21, 19])),
 ]
 
 

@@ -228,6 +228,7 @@ struct PolyconeS {
 // How a kernel obtains a shape's parameters from its block: by value for
 // fixed-size blocks, through a pointer to the device copy for variable-size
 // ones.
+// Length() gives a variable-size block's length in doubles.
 template <class S>
 struct ParamView {
   static constexpr bool kIndirect = false;
@@ -237,6 +238,10 @@ template <>
 struct ParamView<PolyconeS> {
   static constexpr bool kIndirect = true;
   static VGA_HD PolyconeP Get(const double *b) { return PolyconeP{b}; }
+  static VGA_HD int Length(const double *b) {
+    const int n = (int)b[0];
+    return PolyconeS::SecBase(n) + PolyconeS::kSecDoubles * n;
+  }
 };
 
 }  // namespace vga

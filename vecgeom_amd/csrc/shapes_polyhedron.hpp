@@ -18,10 +18,13 @@
 //   [0] nseg  [1] sides  [2] hasPhi  [3] largePhi  [4] bounding-tube z offset
 //   [5] offset of the z planes (nseg + 1)  [6] offset of the phi-section
 //   vectors (3 per side boundary)  [7] offset of the segment records
+//   [8] length of the whole block
 //   [16 ..] TubeP of the bounding tube with its treatment flags fixed as the
 //   reference's HollowTube / NonHollowTube kernels fix them
-//   segment record (8): hasInner, nOuter, nPhi, nInner, offOuter, offPhi,
-//   offInner, unused
+//   segment record (10): hasInner, nOuter, nPhi, nInner, offOuter, offPhi,
+//   offInner, rLo, rHi, unused -- every quadrilateral of the segment lies in
+//   the annular slab Z(k) <= z <= Z(k+1), rLo <= rho <= rHi (rLo: the
+//   smaller inner side distance, rHi: the larger outer corner radius)
 //   quadrilateral record (32): plane normal (3) and distance, four side
 //   planes (normal 3, distance), four corners (3 each)
 #pragma once
@@ -36,7 +39,7 @@ struct PolyhedronP {
 struct PolyhedronS {
   using P = PolyhedronP;
   static constexpr int kTubeOff = 16;
-  static constexpr int kSegDoubles = 8;
+  static constexpr int kSegDoubles = 10;
   static constexpr int kQuadDoubles = 32;
 
   struct Quads {
@@ -67,13 +70,77 @@ struct PolyhedronS {
   static VGA_HD const TubeP &Tube(const P &q) { return *reinterpret_cast<const TubeP *>(q.blk + kTubeOff); }
   static VGA_HD V3 BoundsPoint(const P &q, V3 p) { return V3{p.x, p.y, p.z - q.blk[4]}; }
   static VGA_HD Seg GetSeg(const P &q, int k) {
-    const double *r = q.blk + (int)q.blk[7] + kSegDoubles * k;
+    const double *r = SegRec(q, k);
     Seg s;
     s.hasInner = r[0] != 0;
     s.outer = Quads{q.blk + (int)r[4], (int)r[1]};
     s.phi = Quads{q.blk + (int)r[5], (int)r[2]};
     s.inner = Quads{q.blk + (int)r[6], (int)r[3]};
     return s;
+  }
+
+  static VGA_HD const double *SegRec(const P &q, int k) { return q.blk + (int)q.blk[7] + kSegDoubles * k; }
+
+  // Pruning of whole segments.  The reference visits every segment along the
+  // ray (DistanceToIn) or around the point (safety); a segment is skipped
+  // here only when its result provably cannot change the outcome, so the
+  // results are those of the full walk.
+
+  // False only when no quadrilateral of segment k can yield a hit: a hit is a
+  // point of one of the segment's quadrilaterals (to within rounding), and
+  // those lie in the segment's annular slab, so a ray (t >= 0) that stays
+  // more than kPad (1e-6 mm, far above every rounding of the tests) away from
+  // the slab gets kInfinity from every quadrilateral of the segment.
+  static VGA_HD bool RayNearSegment(const P &q, int k, V3 p, V3 v) {
+    constexpr double kPad = 1e-6;
+    const double *r = SegRec(q, k);
+    const double zlo = Z(q, k) - kPad, zhi = Z(q, k + 1) + kPad;
+    double t0 = 0., t1 = kInfinity;
+    if (v.z != 0) {
+      const double inv = 1. / v.z;
+      double ta = (zlo - p.z) * inv, tb = (zhi - p.z) * inv;
+      if (ta > tb) {
+        const double t = ta;
+        ta = tb;
+        tb = t;
+      }
+      t0 = ta > t0 ? ta : t0;
+      t1 = tb;
+      if (!(t0 <= t1)) return false;
+    } else if (p.z < zlo || p.z > zhi) {
+      return false;
+    }
+    const double a = v.x * v.x + v.y * v.y, b = p.x * v.x + p.y * v.y;
+    double t = a > 0 ? -b / a : t0;
+    t = t < t0 ? t0 : (t > t1 ? t1 : t);
+    const double x = p.x + t * v.x, y = p.y + t * v.y;
+    const double rOut = r[8] + kPad;
+    if (x * x + y * y > rOut * rOut) return false;
+    const double rIn = r[7] - kPad;
+    if (rIn > 0 && t1 < kInfinity) {
+      // rho^2 is convex along the ray: its maximum on [t0, t1] is at an end.
+      const double x0 = p.x + t0 * v.x, y0 = p.y + t0 * v.y, x1 = p.x + t1 * v.x, y1 = p.y + t1 * v.y;
+      const double r0 = x0 * x0 + y0 * y0, r1 = x1 * x1 + y1 * y1;
+      if ((r0 > r1 ? r0 : r1) < rIn * rIn) return false;
+    }
+    return true;
+  }
+
+  // True only when segment k's squared safety cannot be below `safety2`
+  // (a squared distance): every quadrilateral lies in the segment's annular
+  // slab, and the value the reference computes for a quadrilateral is the
+  // squared distance to the quadrilateral (point inside its side planes) or
+  // to one of its edges, never below the squared distance to the slab by
+  // more than rounding; pad covers that rounding with a wide margin.
+  static VGA_HD bool SegmentSafetyAtLeast(const P &q, int k, V3 p, double safety2) {
+    const double *r = SegRec(q, k);
+    const double zlo = Z(q, k), zhi = Z(q, k + 1);
+    const double dz = p.z < zlo ? zlo - p.z : (p.z > zhi ? p.z - zhi : 0.);
+    const double rho = Sqrt(p.x * p.x + p.y * p.y);
+    const double dr = rho < r[7] ? r[7] - rho : (rho > r[8] ? rho - r[8] : 0.);
+    const double scale = Abs(p.x) + Abs(p.y) + Abs(p.z) + r[8] + Abs(zlo) + Abs(zhi);
+    const double lb = Sqrt(dz * dz + dr * dr) - (1e-6 + 1e-8 * scale);
+    return lb > 0 && lb * lb >= safety2;
   }
 
   // DistanceToLineSegmentSquared<kScalar>, GenericKernels.h:118-147
@@ -329,12 +396,12 @@ struct PolyhedronS {
     double distance = kInfinity;
     if (d.z >= 0) {
       for (; k < n; ++k) {
-        distance = DistanceToInSegment(q, k, p, d);
+        distance = RayNearSegment(q, k, p, d) ? DistanceToInSegment(q, k, p, d) : kInfinity;
         if (distance >= 0 && distance < kInfinity) break;
       }
     } else {
       for (; k >= 0; --k) {
-        distance = DistanceToInSegment(q, k, p, d);
+        distance = RayNearSegment(q, k, p, d) ? DistanceToInSegment(q, k, p, d) : kInfinity;
         if (distance >= 0 && distance < kInfinity) break;
       }
     }
@@ -371,12 +438,12 @@ struct PolyhedronS {
     const int k = ClampZ(q, FindZ(q, p.z));
     const int phiIndex = FindPhi(q, p);
     for (int z = k; z < n;) {
-      safety = Min(safety, SafetySegmentSquared(q, z, phiIndex, p));
+      if (!SegmentSafetyAtLeast(q, z, p, safety)) safety = Min(safety, SafetySegmentSquared(q, z, phiIndex, p));
       ++z;
       if (Z(q, z) - p.z > safety) break;
     }
     for (int z = k - 1; z >= 0; --z) {
-      safety = Min(safety, SafetySegmentSquared(q, z, phiIndex, p));
+      if (!SegmentSafetyAtLeast(q, z, p, safety)) safety = Min(safety, SafetySegmentSquared(q, z, phiIndex, p));
       if (p.z - Z(q, z) > safety) break;
     }
     SafetyToEndcapsSquared(q, p, safety);
@@ -390,6 +457,7 @@ template <>
 struct ParamView<PolyhedronS> {
   static constexpr bool kIndirect = true;
   static VGA_HD PolyhedronP Get(const double *b) { return PolyhedronP{b}; }
+  static VGA_HD int Length(const double *b) { return (int)b[8]; }
 };
 
 }  // namespace vga

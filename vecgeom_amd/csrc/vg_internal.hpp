@@ -33,8 +33,10 @@ struct ShapeRec {
   int32_t pad;
 };
 
-// Upper bound on the size of one shape parameter block (doubles).
-constexpr int kMaxParamDoubles = 4096;
+// Upper bound on the size of one shape parameter block (doubles).  Fixed
+// blocks are a few hundred bytes; variable ones (Polycone sections,
+// Polyhedron quadrilaterals) grow with the shape's planes and sides.
+constexpr int kMaxParamDoubles = 1 << 20;
 
 }  // namespace vga
 

@@ -190,7 +190,7 @@ std::vector<double> BuildPolyhedron(const double *c, int n, std::string *err) {
   for (auto &s : sc) nquads += s.nOuter + s.nPhi + s.nInner;
   std::vector<double> blk(quadOff + (size_t)PolyhedronS::kQuadDoubles * nquads, 0.);
   blk[0] = nseg; blk[1] = sides; blk[2] = hasPhi; blk[3] = largePhi; blk[4] = z[0] + 0.5 * tubeZ;
-  blk[5] = zOff; blk[6] = phiOff; blk[7] = segOff;
+  blk[5] = zOff; blk[6] = phiOff; blk[7] = segOff; blk[8] = (double)blk.size();
   std::memcpy(blk.data() + PolyhedronS::kTubeOff, &tube, sizeof(TubeP));
   for (int i = 0; i < nz; ++i) blk[zOff + i] = z[i];
   for (int i = 0; i <= sides; ++i) {
@@ -215,6 +215,10 @@ std::vector<double> BuildPolyhedron(const double *c, int n, std::string *err) {
   for (int ip = 0; ip < nseg; ++ip) {
     double *rec = blk.data() + segOff + PolyhedronS::kSegDoubles * ip;
     rec[0] = sc[ip].inner; rec[1] = sc[ip].nOuter; rec[2] = sc[ip].nPhi; rec[3] = sc[ip].nInner;
+    // Annular slab holding the segment's quadrilaterals: inner side distance
+    // and outer corner radius.
+    rec[7] = sc[ip].inner ? std::min(rmin[ip], rmin[ip + 1]) : 0.;
+    rec[8] = std::max(std::max(rmax[ip], rmax[ip + 1]), std::max(rmin[ip], rmin[ip + 1])) / cosHalf;
     rec[4] = quadOff;
     for (int k = 0; k < sc[ip].nOuter; ++k) {
       double *q = blk.data() + quadOff + PolyhedronS::kQuadDoubles * k;

@@ -69,8 +69,21 @@ __device__ __forceinline__ void Eval(const typename S::P &P, const Xf &X, V3 p, 
 constexpr bool NeedsDir(int q) { return q == kQDistIn || q == kQDistInMin || q == kQDistOut || q == kQPlacedDistOut; }
 constexpr bool NeedsStep(int q) { return q == kQDistIn || q == kQDistOut || q == kQPlacedDistOut; }
 
-template <class S, int XK, int Q, int IT>
-__global__ __launch_bounds__(kBlock) void ShapeQueryKernel(const typename S::P P, const Xf X, const QArgs a) {
+// Variable-length parameter blocks (Polycone, Polyhedron) are read many
+// times per query; with Stage the workgroup first copies the block to LDS
+// and every lane reads it from there.
+constexpr int kMaxStageDoubles = 6144;  // 48 KiB of LDS
+
+template <class S, int XK, int Q, int IT, bool Stage>
+__global__ __launch_bounds__(kBlock) void ShapeQueryKernel(const typename S::P Pg, const Xf X, const QArgs a,
+                                                           int stage_len) {
+  typename S::P P = Pg;
+  if constexpr (Stage) {
+    extern __shared__ double stage[];
+    for (int k = threadIdx.x; k < stage_len; k += kBlock) stage[k] = Pg.blk[k];
+    __syncthreads();
+    P = typename S::P{stage};
+  }
   const uint64_t i0 = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * IT;
   if (i0 >= a.n) return;
   double px[IT], py[IT], pz[IT], dx[IT], dy[IT], dz[IT], sm[IT];
@@ -216,29 +229,39 @@ bool Aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) ==
 int g_variant = 0;  // 0 = tuned kernels, 1 = plain FP64 form
 
 template <class S, int XK, int Q>
-int LaunchQ(const typename S::P &P, const Xf &X, const QArgs &a, hipStream_t s) {
+int LaunchQ(const typename S::P &P, const Xf &X, const QArgs &a, int stage_len, hipStream_t s) {
   if (a.n == 0) return VG_OK;
   bool al = Aligned16(a.x) && Aligned16(a.y) && Aligned16(a.z) && Aligned16(a.out);
   if (NeedsDir(Q)) al = al && Aligned16(a.dx) && Aligned16(a.dy) && Aligned16(a.dz);
   if (NeedsStep(Q) && a.sm) al = al && Aligned16(a.sm);
+  if constexpr (ParamView<S>::kIndirect) {
+    if (stage_len > 0 && stage_len <= kMaxStageDoubles) {
+      const uint64_t blocks = (a.n + kBlock - 1) / kBlock;
+      hipLaunchKernelGGL((ShapeQueryKernel<S, XK, Q, 1, true>), dim3((unsigned)blocks), dim3(kBlock),
+                         stage_len * sizeof(double), s, P, X, a, stage_len);
+      return CheckHip(hipGetLastError(), "shape query launch");
+    }
+  }
   if (al) {
     constexpr int IT = 2;
     const uint64_t blocks = (a.n + (uint64_t)kBlock * IT - 1) / ((uint64_t)kBlock * IT);
-    hipLaunchKernelGGL((ShapeQueryKernel<S, XK, Q, IT>), dim3((unsigned)blocks), dim3(kBlock), 0, s, P, X, a);
+    hipLaunchKernelGGL((ShapeQueryKernel<S, XK, Q, IT, false>), dim3((unsigned)blocks), dim3(kBlock), 0, s, P, X, a,
+                       0);
   } else {
     const uint64_t blocks = (a.n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL((ShapeQueryKernel<S, XK, Q, 1>), dim3((unsigned)blocks), dim3(kBlock), 0, s, P, X, a);
+    hipLaunchKernelGGL((ShapeQueryKernel<S, XK, Q, 1, false>), dim3((unsigned)blocks), dim3(kBlock), 0, s, P, X, a,
+                       0);
   }
   return CheckHip(hipGetLastError(), "shape query launch");
 }
 
 template <class S, int Q>
-int LaunchXf(const typename S::P &P, const PlacedRec &pr, const QArgs &a, hipStream_t s) {
+int LaunchXf(const typename S::P &P, const PlacedRec &pr, const QArgs &a, int stage_len, hipStream_t s) {
   switch (pr.xfkind) {
-    case kXfIdentity: return LaunchQ<S, kXfIdentity, Q>(P, pr.xf, a, s);
-    case kXfTranslation: return LaunchQ<S, kXfTranslation, Q>(P, pr.xf, a, s);
-    case kXfRotation: return LaunchQ<S, kXfRotation, Q>(P, pr.xf, a, s);
-    default: return LaunchQ<S, kXfGeneral, Q>(P, pr.xf, a, s);
+    case kXfIdentity: return LaunchQ<S, kXfIdentity, Q>(P, pr.xf, a, stage_len, s);
+    case kXfTranslation: return LaunchQ<S, kXfTranslation, Q>(P, pr.xf, a, stage_len, s);
+    case kXfRotation: return LaunchQ<S, kXfRotation, Q>(P, pr.xf, a, stage_len, s);
+    default: return LaunchQ<S, kXfGeneral, Q>(P, pr.xf, a, stage_len, s);
   }
 }
 
@@ -271,22 +294,24 @@ int LaunchShape(const vg_geometry *g, int32_t pid, const double *block, const Pl
     return LaunchShapePlain<S>(g, pid, q, a, s);
   }
   typename S::P P;
+  int len = 0;
   if constexpr (ParamView<S>::kIndirect) {
     P = ParamView<S>::Get(g->d_params + pr.param_off);
+    len = ParamView<S>::Length(block);
   } else {
     std::memcpy(&P, block, sizeof(P));
   }
   switch (q) {
-    case kQContains: return LaunchXf<S, kQContains>(P, pr, a, s);
-    case kQInside: return LaunchXf<S, kQInside>(P, pr, a, s);
-    case kQDistIn: return LaunchXf<S, kQDistIn>(P, pr, a, s);
-    case kQDistInMin: return LaunchXf<S, kQDistInMin>(P, pr, a, s);
-    case kQDistOut: return LaunchQ<S, kXfIdentity, kQDistOut>(P, pr.xf, a, s);
-    case kQPlacedDistOut: return LaunchXf<S, kQPlacedDistOut>(P, pr, a, s);
-    case kQSafeIn: return LaunchXf<S, kQSafeIn>(P, pr, a, s);
-    case kQSafeInMin: return LaunchXf<S, kQSafeInMin>(P, pr, a, s);
-    case kQSafeOut: return LaunchQ<S, kXfIdentity, kQSafeOut>(P, pr.xf, a, s);
-    case kQSafeOutMin: return LaunchQ<S, kXfIdentity, kQSafeOutMin>(P, pr.xf, a, s);
+    case kQContains: return LaunchXf<S, kQContains>(P, pr, a, len, s);
+    case kQInside: return LaunchXf<S, kQInside>(P, pr, a, len, s);
+    case kQDistIn: return LaunchXf<S, kQDistIn>(P, pr, a, len, s);
+    case kQDistInMin: return LaunchXf<S, kQDistInMin>(P, pr, a, len, s);
+    case kQDistOut: return LaunchQ<S, kXfIdentity, kQDistOut>(P, pr.xf, a, len, s);
+    case kQPlacedDistOut: return LaunchXf<S, kQPlacedDistOut>(P, pr, a, len, s);
+    case kQSafeIn: return LaunchXf<S, kQSafeIn>(P, pr, a, len, s);
+    case kQSafeInMin: return LaunchXf<S, kQSafeInMin>(P, pr, a, len, s);
+    case kQSafeOut: return LaunchQ<S, kXfIdentity, kQSafeOut>(P, pr.xf, a, len, s);
+    case kQSafeOutMin: return LaunchQ<S, kXfIdentity, kQSafeOutMin>(P, pr.xf, a, len, s);
   }
   return SetError(VG_ERR_INVALID, "unknown query");
 }
