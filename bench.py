@@ -32,6 +32,26 @@ BYTES = {"contains": 25, "inside": 28, "dist_in": 64, "dist_out": 64, "safe_in":
 BYTES_F32 = {"contains": 13, "inside": 16, "dist_in": 40, "dist_out": 40, "safe_in": 20, "safe_out": 20}
 
 
+PROFILE = os.path.join(ROOT, "profiles", "shape_zoo_fp64_kernels.md")
+KERNEL_SHAPE = {"Parallelepiped": "Para", "Paraboloid": "Parab", "Trapezoid": "Trap"}
+
+
+def profiled_traffic(shape, query, n):
+    """HBM bytes per launch of a shape kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (bytes per query, scaled by a device copy
+    of known size) times this launch's queries; None if not profiled."""
+    key = f"{KERNEL_SHAPE.get(shape, shape)}/{query}"
+    try:
+        with open(PROFILE) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) == 7 and cells[1] == key:
+                    return int(round(float(cells[6]) * n))
+    except OSError:
+        pass
+    return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -356,8 +376,8 @@ def serve_bench(args, dev):
     Latency is per request, submission to completion.  The batched form
     merges all pending requests into one launch on each of two streams; the
     per-basket form launches each request alone, one at a time.  Rates are
-    fractions of the per-basket form's saturation rate (all baskets arriving
-    at once)."""
+    0.25, 0.5 and 0.9 of the per-basket form's saturation rate and 0.5 and
+    0.9 of the batched form's (saturation: all baskets arriving at once)."""
     from vecgeom_amd import geometry as G
     from vecgeom_amd import serve as S
     from vecgeom_amd import workloads as W
@@ -388,9 +408,9 @@ def serve_bench(args, dev):
     run("batched", zero)  # warm-up
     sat = {f: run(f, zero) for f in forms}
     base_rate = nb / sat["per_basket"]["wall_s"]
+    batched_rate = nb / sat["batched"]["wall_s"]
     rates = {}
-    for frac in (0.25, 0.5, 0.9):
-        rate = frac * base_rate
+    for rate in [f * base_rate for f in (0.25, 0.5, 0.9)] + [f * batched_rate for f in (0.5, 0.9)]:
         gaps = rng.exponential(1.0 / rate, nb)
         arrival = np.cumsum(gaps) - gaps[0]
         rates[f"{rate:.0f}/s"] = {f: run(f, arrival) for f in forms}
@@ -398,6 +418,7 @@ def serve_bench(args, dev):
                        f"({int(sizes.sum())} tracks), {steps} steps per basket, Poisson arrivals, 32 GiB arena",
            "unit_steps": "G steps/s", "saturation": sat,
            "saturation_basket_rate_per_basket_form": round(base_rate, 1),
+           "saturation_basket_rate_batched_form": round(batched_rate, 1),
            "fixed_rates": rates, "dtype": "f64"}
     srv.destroy()
     geom.free()
@@ -491,7 +512,7 @@ def main():
     dom_bytes = dom["n"] * BYTES[dom["query"]]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom["shape"], dom["query"], dom["n"]),
             "kernel": f"ShapeQueryKernel<{dom['shape']},{dom['query']}>", "avg_launch_us": round(1e3 * dom_ms, 3),
             "bytes_per_query": BYTES[dom["query"]], "queries_per_launch": dom["n"], "bytes_per_launch": dom_bytes}
 

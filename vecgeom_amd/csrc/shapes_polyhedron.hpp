@@ -25,8 +25,8 @@
 //   offInner, rLo, rHi, unused -- every quadrilateral of the segment lies in
 //   the annular slab Z(k) <= z <= Z(k+1), rLo <= rho <= rHi (rLo: the
 //   smaller inner side distance, rHi: the larger outer corner radius)
-//   quadrilateral record (32): plane normal (3) and distance, four side
-//   planes (normal 3, distance), four corners (3 each)
+//   quadrilateral record (33): plane normal (3) and distance, four side
+//   planes (normal 3, distance), four corners (3 each), unused
 #pragma once
 #include "shapes_polycone.hpp"
 
@@ -40,7 +40,10 @@ struct PolyhedronS {
   using P = PolyhedronP;
   static constexpr int kTubeOff = 16;
   static constexpr int kSegDoubles = 10;
-  static constexpr int kQuadDoubles = 32;
+  // 33, not 32: with a record stride of 64 dwords every quadrilateral's
+  // field k would sit in the same LDS bank, and lanes reading different
+  // quadrilaterals would conflict.
+  static constexpr int kQuadDoubles = 33;
 
   struct Quads {
     const double *q;  // first record

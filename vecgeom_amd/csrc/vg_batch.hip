@@ -10,15 +10,19 @@
 // task record, the shape parameters and the placement are uniform per tile
 // (scalar loads), and the switch over shape and query is a uniform branch.
 //
-// Shapes are split into three weight classes by the registers their kernels
-// need (light: Box, Tube, Orb, Trd, Trapezoid, Parallelepiped, Paraboloid;
-// medium: Cone, Sphere, Polycone; heavy: Torus).  Each class has its own
-// kernel, so a Box tile is not held to the register budget of the Torus
-// quartic; a batch costs one launch per class it uses.
+// Shapes are split into weight classes by the registers their kernels need
+// (light: Box, Tube, Orb, Trd, Trapezoid, Parallelepiped, Paraboloid;
+// medium: Cone, Sphere; heavy: Torus; variable-length parameter blocks:
+// Polycone, Polyhedron -- staged in LDS per workgroup when the block fits,
+// read from global memory otherwise).  Each class has its own kernel, so a
+// Box tile is not held to the register budget of the Torus quartic; a batch
+// costs one launch per class it uses, and the class kernels run
+// concurrently.
 //
-// Launch shapes: grid = one workgroup per tile, or a persistent grid of
-// `blocks` workgroups striding over the tiles; IT = 1, 2 or 4 items per lane.
-// vg_batch_autotune picks both by timing the batch itself.
+// Launch shapes: IT = 1, 2 or 4 items per lane (at most 2 for the heavy
+// classes); the light class runs one workgroup per tile or a persistent grid
+// of `blocks` workgroups striding over the tiles, the others one workgroup
+// per tile.  vg_batch_autotune picks IT and the grid by timing the batch.
 //
 // Inputs may be double (the FP64 form) or float (FP32 storage: each value is
 // promoted to double in registers, so results equal the FP64 kernels on the
@@ -43,6 +47,7 @@ constexpr int kBatchBlock = 256;
 struct TaskDev {
   int32_t type, query, xfkind, in_type;
   int32_t param_off, vec_ok;
+  int32_t param_len, pad;  // parameter block length (doubles)
   uint64_t n, tile_begin;
   const void *x, *y, *z, *dx, *dy, *dz;
   const double *sm;
@@ -77,9 +82,10 @@ __device__ __forceinline__ void LoadAll(const TaskDev t, uint64_t i0, bool full,
     Load<double, IT>(a, i0, full, vec, t.n, o);
 }
 
+// blk: the task's parameter block (in global memory or staged in LDS).
 template <class S, int Q, int IT>
-__device__ __forceinline__ void RunTile(const TaskDev t, const double *params, uint64_t i0) {
-  const auto &P = ParamView<S>::Get(params + t.param_off);
+__device__ __forceinline__ void RunTile(const TaskDev t, const double *blk, uint64_t i0) {
+  const auto &P = ParamView<S>::Get(blk);
   const bool full = i0 + IT <= t.n;
   constexpr bool dir = (Q == kQDistIn || Q == kQDistOut || Q == kQPlacedDistOut);
   double px[IT], py[IT], pz[IT], dx[IT], dy[IT], dz[IT], sm[IT];
@@ -121,47 +127,66 @@ __device__ __forceinline__ void RunTile(const TaskDev t, const double *params, u
 }
 
 template <class S, int IT>
-__device__ __forceinline__ void RunShape(const TaskDev t, const double *params, uint64_t i0) {
+__device__ __forceinline__ void RunShape(const TaskDev t, const double *blk, uint64_t i0) {
   switch (t.query) {
-    case kQContains: RunTile<S, kQContains, IT>(t, params, i0); break;
-    case kQInside: RunTile<S, kQInside, IT>(t, params, i0); break;
-    case kQDistIn: RunTile<S, kQDistIn, IT>(t, params, i0); break;
-    case kQDistOut: RunTile<S, kQDistOut, IT>(t, params, i0); break;
-    case kQPlacedDistOut: RunTile<S, kQPlacedDistOut, IT>(t, params, i0); break;
-    case kQSafeIn: RunTile<S, kQSafeIn, IT>(t, params, i0); break;
-    case kQSafeOut: RunTile<S, kQSafeOut, IT>(t, params, i0); break;
+    case kQContains: RunTile<S, kQContains, IT>(t, blk, i0); break;
+    case kQInside: RunTile<S, kQInside, IT>(t, blk, i0); break;
+    case kQDistIn: RunTile<S, kQDistIn, IT>(t, blk, i0); break;
+    case kQDistOut: RunTile<S, kQDistOut, IT>(t, blk, i0); break;
+    case kQPlacedDistOut: RunTile<S, kQPlacedDistOut, IT>(t, blk, i0); break;
+    case kQSafeIn: RunTile<S, kQSafeIn, IT>(t, blk, i0); break;
+    case kQSafeOut: RunTile<S, kQSafeOut, IT>(t, blk, i0); break;
   }
 }
 
-constexpr int kClasses = 3;
+// Classes 3 and 4 hold the shapes with variable-length parameter blocks:
+// class 3 those whose block fits in LDS (staged per workgroup), class 4 the
+// others (read from global memory).
+constexpr int kClasses = 5;
+constexpr int kStagedClass = 3;
+constexpr int kMaxStageDoubles = 6144;  // 48 KiB of LDS
 
 constexpr int ClassOf(int type) {
-  return (type == VG_CONE || type == VG_SPHERE || type == VG_POLYCONE || type == VG_POLYHEDRON) ? 1
-                                                                                                 : (type == VG_TORUS ? 2 : 0);
+  return (type == VG_CONE || type == VG_SPHERE)           ? 1
+         : (type == VG_TORUS)                              ? 2
+         : (type == VG_POLYCONE || type == VG_POLYHEDRON) ? 3
+                                                           : 0;
 }
 
+// Items per lane of class c when the batch asks for `it`: the light class
+// takes it as given, the compute-heavy classes at most 2.
+constexpr int ClassIT(int c, int it) { return c == 0 ? it : (it < 2 ? it : 2); }
+
 template <int IT, int CLS>
-__device__ __forceinline__ void RunTask(const TaskDev t, const double *params, uint64_t i0) {
+__device__ __forceinline__ void RunTask(const TaskDev t, const double *blk, uint64_t i0) {
   if constexpr (CLS == 0) {
     switch (t.type) {
-      case VG_BOX: RunShape<BoxS, IT>(t, params, i0); break;
-      case VG_TUBE: RunShape<TubeS, IT>(t, params, i0); break;
-      case VG_ORB: RunShape<OrbS, IT>(t, params, i0); break;
-      case VG_TRD: RunShape<TrdS, IT>(t, params, i0); break;
-      case VG_TRAPEZOID: RunShape<TrapS, IT>(t, params, i0); break;
-      case VG_PARALLELEPIPED: RunShape<ParaS, IT>(t, params, i0); break;
-      case VG_PARABOLOID: RunShape<ParabS, IT>(t, params, i0); break;
+      case VG_BOX: RunShape<BoxS, IT>(t, blk, i0); break;
+      case VG_TUBE: RunShape<TubeS, IT>(t, blk, i0); break;
+      case VG_ORB: RunShape<OrbS, IT>(t, blk, i0); break;
+      case VG_TRD: RunShape<TrdS, IT>(t, blk, i0); break;
+      case VG_TRAPEZOID: RunShape<TrapS, IT>(t, blk, i0); break;
+      case VG_PARALLELEPIPED: RunShape<ParaS, IT>(t, blk, i0); break;
+      case VG_PARABOLOID: RunShape<ParabS, IT>(t, blk, i0); break;
     }
   } else if constexpr (CLS == 1) {
     switch (t.type) {
-      case VG_SPHERE: RunShape<SphereS, IT>(t, params, i0); break;
-      case VG_CONE: RunShape<ConeS, IT>(t, params, i0); break;
-      case VG_POLYCONE: RunShape<PolyconeS, IT>(t, params, i0); break;
-      case VG_POLYHEDRON: RunShape<PolyhedronS, IT>(t, params, i0); break;
+      case VG_SPHERE: RunShape<SphereS, IT>(t, blk, i0); break;
+      case VG_CONE: RunShape<ConeS, IT>(t, blk, i0); break;
     }
-  } else {
-    if (t.type == VG_TORUS) RunShape<TorusS, IT>(t, params, i0);
+  } else if constexpr (CLS == 2) {
+    if (t.type == VG_TORUS) RunShape<TorusS, IT>(t, blk, i0);
+  } else {  // classes 3 and 4
+    switch (t.type) {
+      case VG_POLYCONE: RunShape<PolyconeS, IT>(t, blk, i0); break;
+      case VG_POLYHEDRON: RunShape<PolyhedronS, IT>(t, blk, i0); break;
+    }
   }
+}
+
+// Copies a task's parameter block to LDS (whole workgroup).
+__device__ __forceinline__ void Stage(double *stage, const double *src, int len) {
+  for (int j = threadIdx.x; j < len; j += kBatchBlock) stage[j] = src[j];
 }
 
 // Tile -> task: the last task whose first tile is <= tile (tasks sorted by
@@ -178,14 +203,34 @@ __device__ __forceinline__ int FindTask(const TaskDev *tasks, int ntasks, uint64
   return lo;
 }
 
+// The light class runs on a persistent grid if asked (blocks striding over
+// the tiles).  The compute-heavy classes take one tile per workgroup: a
+// persistent loop around their bodies keeps the task's uniform values live
+// across iterations and more than doubles their VGPR count.
 template <int IT, int CLS>
 __global__ __launch_bounds__(kBatchBlock) void BatchKernel(const TaskDev *__restrict__ tasks, int ntasks,
                                                            uint64_t ntiles, const double *__restrict__ params) {
-  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  if constexpr (CLS == 0) {
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+      const int k = FindTask(tasks, ntasks, tile);
+      const TaskDev t = tasks[k];
+      const uint64_t i0 = ((tile - t.tile_begin) * kBatchBlock + threadIdx.x) * IT;
+      if (i0 < t.n) RunTask<IT, CLS>(t, params + t.param_off, i0);
+    }
+  } else {
+    const uint64_t tile = blockIdx.x;
+    if (tile >= ntiles) return;
     const int k = FindTask(tasks, ntasks, tile);
     const TaskDev t = tasks[k];
     const uint64_t i0 = ((tile - t.tile_begin) * kBatchBlock + threadIdx.x) * IT;
-    if (i0 < t.n) RunTask<IT, CLS>(t, params, i0);
+    if constexpr (CLS == kStagedClass) {
+      extern __shared__ double stage[];
+      Stage(stage, params + t.param_off, t.param_len);
+      __syncthreads();
+      if (i0 < t.n) RunTask<IT, CLS>(t, stage, i0);
+    } else {
+      if (i0 < t.n) RunTask<IT, CLS>(t, params + t.param_off, i0);
+    }
   }
 }
 
@@ -194,7 +239,16 @@ __global__ __launch_bounds__(kBatchBlock) void BatchKernel(const TaskDev *__rest
 template <int IT, int CLS>
 __global__ __launch_bounds__(kBatchBlock) void TaskKernel(const TaskDev t, const double *__restrict__ params) {
   const uint64_t i0 = ((uint64_t)blockIdx.x * kBatchBlock + threadIdx.x) * IT;
-  if (i0 < t.n) RunTask<IT, CLS>(t, params, i0);
+  if constexpr (CLS == kStagedClass) {
+    extern __shared__ double stage[];
+    if (t.param_len <= kMaxStageDoubles) {
+      Stage(stage, params + t.param_off, t.param_len);
+      __syncthreads();
+      if (i0 < t.n) RunTask<IT, CLS>(t, stage, i0);
+      return;
+    }
+  }
+  if (i0 < t.n) RunTask<IT, CLS>(t, params + t.param_off, i0);
 }
 
 }  // namespace
@@ -207,8 +261,9 @@ struct vg_batch {
   const vg_geometry *g = nullptr;
   std::vector<vg_query_task> tasks;
   std::vector<TaskDev> host;                  // sorted by weight class
-  int first[vga::kClasses + 1] = {0, 0, 0, 0};  // class c owns host[first[c] .. first[c+1])
-  uint64_t ntiles[vga::kClasses] = {0, 0, 0};   // tiles per class for the current IT
+  int first[vga::kClasses + 1] = {0, 0, 0, 0, 0, 0};  // class c owns host[first[c] .. first[c+1])
+  uint64_t ntiles[vga::kClasses] = {0, 0, 0, 0, 0};   // tiles per class for the current IT
+  int stage_len = 0;                               // LDS doubles for the staged class
   TaskDev *d_tasks = nullptr;
   int it = 2;            // items per lane
   unsigned blocks = 0;   // 0 = one workgroup per tile, else a persistent grid per class
@@ -216,9 +271,9 @@ struct vg_batch {
   // Classes run concurrently: the first non-empty class on the caller's
   // stream, the others on side streams forked from and joined back to it
   // with events (a pattern hipGraph capture records as parallel branches).
-  hipStream_t side[vga::kClasses - 1] = {nullptr, nullptr};
+  hipStream_t side[vga::kClasses - 1] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t fork = nullptr;
-  hipEvent_t join[vga::kClasses - 1] = {nullptr, nullptr};
+  hipEvent_t join[vga::kClasses - 1] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -240,14 +295,20 @@ int QueryCode(int32_t q) {
   return -1;
 }
 
+int TaskClass(const TaskDev &t) {
+  const int c = ClassOf(t.type);
+  return (c == kStagedClass && t.param_len > kMaxStageDoubles) ? kStagedClass + 1 : c;
+}
+
 // Fills tile_begin (numbered within each class) for items per lane `it`.
 void Schedule(vg_batch *b) {
   for (int c = 0; c < kClasses; ++c) {
     uint64_t tile = 0;
+    const uint64_t per = (uint64_t)kBatchBlock * ClassIT(c, b->it);
     for (int k = b->first[c]; k < b->first[c + 1]; ++k) {
       TaskDev &x = b->host[k];
       x.tile_begin = tile;
-      tile += (x.n + (uint64_t)kBatchBlock * b->it - 1) / ((uint64_t)kBatchBlock * b->it);
+      tile += (x.n + per - 1) / per;
     }
     b->ntiles[c] = tile;
   }
@@ -255,17 +316,19 @@ void Schedule(vg_batch *b) {
 
 template <int IT, int CLS>
 void LaunchClass(const vg_batch *b, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL((BatchKernel<IT, CLS>), dim3(grid), dim3(kBatchBlock), 0, s, b->d_tasks + b->first[CLS],
+  const size_t lds = CLS == kStagedClass ? (size_t)b->stage_len * sizeof(double) : 0;
+  hipLaunchKernelGGL((BatchKernel<IT, CLS>), dim3(grid), dim3(kBatchBlock), lds, s, b->d_tasks + b->first[CLS],
                      b->first[CLS + 1] - b->first[CLS], b->ntiles[CLS], b->g->d_params);
 }
 
 template <int CLS>
 void LaunchIT(const vg_batch *b, unsigned grid, hipStream_t s) {
-  if (b->it == 1)
+  const int it = ClassIT(CLS, b->it);
+  if (it == 1)
     LaunchClass<1, CLS>(b, grid, s);
-  else if (b->it == 2)
+  else if (it == 2)
     LaunchClass<2, CLS>(b, grid, s);
-  else
+  else if constexpr (ClassIT(CLS, 4) == 4)
     LaunchClass<4, CLS>(b, grid, s);
 }
 
@@ -287,13 +350,17 @@ int Launch(vg_batch *b, hipStream_t s) {
       if (rc) break;
     }
     first = false;
-    const unsigned grid = b->blocks ? (unsigned)std::min<uint64_t>(b->blocks, nt) : (unsigned)nt;
+    const unsigned grid = (c == 0 && b->blocks) ? (unsigned)std::min<uint64_t>(b->blocks, nt) : (unsigned)nt;
     if (c == 0)
       LaunchIT<0>(b, grid, cs);
     else if (c == 1)
       LaunchIT<1>(b, grid, cs);
-    else
+    else if (c == 2)
       LaunchIT<2>(b, grid, cs);
+    else if (c == 3)
+      LaunchIT<3>(b, grid, cs);
+    else
+      LaunchIT<4>(b, grid, cs);
     rc = CheckHip(hipGetLastError(), "batch launch");
     if (!rc && cs != s) {
       rc = CheckHip(hipEventRecord(b->join[nside], cs), "batch join");
@@ -351,6 +418,10 @@ int MakeTask(const vg_geometry *g, const vg_query_task &q, int i, TaskDev &t) {
   t.xfkind = pr.xfkind;
   t.in_type = q.input_type;
   t.param_off = pr.param_off;
+  const double *blk = g->params.data() + pr.param_off;
+  t.param_len = pr.type == VG_POLYCONE     ? ParamView<PolyconeS>::Length(blk)
+                : pr.type == VG_POLYHEDRON ? ParamView<PolyhedronS>::Length(blk)
+                                           : 0;
   t.n = q.n;
   t.x = q.x;
   t.y = q.y;
@@ -394,12 +465,14 @@ int vg_batch_create(const vg_geometry *g, const vg_query_task *tasks, int32_t nt
     b->host.push_back(t);
   }
   std::stable_sort(b->host.begin(), b->host.end(),
-                   [](const TaskDev &a, const TaskDev &c) { return ClassOf(a.type) < ClassOf(c.type); });
+                   [](const TaskDev &a, const TaskDev &c) { return TaskClass(a) < TaskClass(c); });
   for (int c = 0; c <= kClasses; ++c) {
     int k = 0;
-    while (k < (int)b->host.size() && ClassOf(b->host[k].type) < c) ++k;
+    while (k < (int)b->host.size() && TaskClass(b->host[k]) < c) ++k;
     b->first[c] = k;
   }
+  for (int k = b->first[kStagedClass]; k < b->first[kStagedClass + 1]; ++k)
+    b->stage_len = std::max(b->stage_len, b->host[k].param_len);
   rc = CreateSync(b);
   if (!rc && !b->host.empty()) {
     rc = CheckHip(hipMalloc(&b->d_tasks, b->host.size() * sizeof(TaskDev)), "hipMalloc(batch table)");
@@ -426,13 +499,16 @@ int vg_query(const vg_geometry *g, const vg_query_task *task, void *stream) {
   const uint64_t blocks = (t.n + (uint64_t)it * kBatchBlock - 1) / ((uint64_t)it * kBatchBlock);
   const hipStream_t s = (hipStream_t)stream;
   const dim3 gr((unsigned)blocks), bl(kBatchBlock);
+  const size_t lds = t.param_len <= kMaxStageDoubles ? (size_t)t.param_len * sizeof(double) : 0;
   switch (ClassOf(t.type) * 2 + (it == 2 ? 1 : 0)) {
     case 0: hipLaunchKernelGGL((TaskKernel<1, 0>), gr, bl, 0, s, t, g->d_params); break;
     case 1: hipLaunchKernelGGL((TaskKernel<2, 0>), gr, bl, 0, s, t, g->d_params); break;
     case 2: hipLaunchKernelGGL((TaskKernel<1, 1>), gr, bl, 0, s, t, g->d_params); break;
     case 3: hipLaunchKernelGGL((TaskKernel<2, 1>), gr, bl, 0, s, t, g->d_params); break;
     case 4: hipLaunchKernelGGL((TaskKernel<1, 2>), gr, bl, 0, s, t, g->d_params); break;
-    default: hipLaunchKernelGGL((TaskKernel<2, 2>), gr, bl, 0, s, t, g->d_params); break;
+    case 5: hipLaunchKernelGGL((TaskKernel<2, 2>), gr, bl, 0, s, t, g->d_params); break;
+    case 6: hipLaunchKernelGGL((TaskKernel<1, 3>), gr, bl, lds, s, t, g->d_params); break;
+    default: hipLaunchKernelGGL((TaskKernel<2, 3>), gr, bl, lds, s, t, g->d_params); break;
   }
   return CheckHip(hipGetLastError(), "query launch");
 }
