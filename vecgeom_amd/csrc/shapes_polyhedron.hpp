@@ -388,6 +388,16 @@ struct PolyhedronS {
     if (OnEndcap(q, GetSeg(q, isFirst ? 0 : n - 1), V3{p.x, p.y, p.z + t})) distSq = tSq;
   }
 
+  // The bounding-tube rejection at the head of DistanceToIn: true (and
+  // out = kInfinity) exactly when DistanceToIn returns kInfinity there.
+  static constexpr bool kHasQuick = true;
+  static VGA_HD bool DistanceToInQuick(const P &q, V3 p, V3 d, double stepMax, double &out) {
+    const V3 bp = BoundsPoint(q, p);
+    if (TubeS::Contains(Tube(q), bp) || TubeS::DistanceToIn(Tube(q), bp, d, stepMax) != kInfinity) return false;
+    out = kInfinity;
+    return true;
+  }
+
   // ScalarDistanceToInKernel, :832-913 (local point and direction)
   static VGA_HD double DistanceToIn(const P &q, V3 p, V3 d, double stepMax) {
     const V3 bp = BoundsPoint(q, p);

@@ -212,6 +212,15 @@ struct TorusS {
     return current;
   }
 
+  // The bounding-tube rejection at the head of DistanceToIn: true (and
+  // out = kInfinity) exactly when DistanceToIn returns kInfinity there.
+  static constexpr bool kHasQuick = true;
+  static VGA_HD bool DistanceToInQuick(const P &t, V3 p, V3 v, double stepMax, double &out) {
+    if (TubeS::Contains(t.btube, p) || TubeS::DistanceToIn(t.btube, p, v, stepMax) != kInfinity) return false;
+    out = kInfinity;
+    return true;
+  }
+
   static VGA_HD double DistanceToIn(const P &t, V3 p, V3 v, double stepMax) {
     const bool inBounds = TubeS::Contains(t.btube, p);
     const double tubeDistance = TubeS::DistanceToIn(t.btube, p, v, stepMax);

@@ -173,6 +173,7 @@ int BuildGeometry(const vg_flat_geometry *f, vg_geometry *g) {
     }
     ComputeAABB(lo, hi, r.xf, g->aabb.data() + 6 * (size_t)p);
     if (!g->nav_bad_type && !ShapeHasKernels(r.type)) g->nav_bad_type = r.type;
+    if (r.type >= 0 && r.type < 32) g->type_mask |= 1u << r.type;
   }
   g->max_depth = Depth(g, g->world, 1, 64);
   if (g->max_depth < 0) {

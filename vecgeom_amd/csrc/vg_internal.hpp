@@ -57,6 +57,7 @@ struct vg_geometry {
   int32_t *d_daughters = nullptr;
   float *d_aabb = nullptr;  // 6 floats per placed (ABBoxManager keeps floats)
   int32_t nav_bad_type = 0;  // first shape type in the tree without device kernels (0: none)
+  uint32_t type_mask = 0;    // bit t set: some placed volume has shape type t
 };
 
 namespace vga {

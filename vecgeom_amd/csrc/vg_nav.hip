@@ -82,7 +82,7 @@ VGA_NAV void StoreState(const vg_nav_states &s, uint64_t i, const DPath<D> &p) {
   for (int l = 0; l < s.depth; ++l) s.path[(uint64_t)l * s.capacity + i] = l < D ? p.Get(l) : -1;
 }
 
-template <int D>
+template <int D, uint32_t M>
 __global__ __launch_bounds__(kNavBlock) void LocateKernel(const NavDev g, const double *__restrict__ x,
                                                           const double *__restrict__ y, const double *__restrict__ z,
                                                           uint64_t n, const vg_nav_states s) {
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kNavBlock) void LocateKernel(const NavDev g, const 
   if (i >= n) return;
   DPath<D> p;
   p.Clear();
-  LocatePoint(g, g.world, V3{x[i], y[i], z[i]}, p, true);
+  LocatePoint<D, M>(g, g.world, V3{x[i], y[i], z[i]}, p, true);
   StoreState(s, i, p);
 }
 
@@ -102,7 +102,7 @@ struct StepArgs {
   int32_t pad;
 };
 
-template <int D>
+template <int D, uint32_t M>
 __global__ __launch_bounds__(kNavBlock) void StepKernel(const NavDev g, const StepArgs a, const vg_nav_states cur,
                                                         const vg_nav_states nxt) {
   const uint64_t i = (uint64_t)blockIdx.x * kNavBlock + threadIdx.x;
@@ -112,7 +112,8 @@ __global__ __launch_bounds__(kNavBlock) void StepKernel(const NavDev g, const St
   double step = 0;
   const double ps = a.pstep ? a.pstep[i] : kInfinity;
   if (ValidState(g, c)) {
-    FindNextBoundaryAndStep(g, V3{a.x[i], a.y[i], a.z[i]}, V3{a.dx[i], a.dy[i], a.dz[i]}, c, o, ps, step, a.kind);
+    FindNextBoundaryAndStep<D, false, M>(g, V3{a.x[i], a.y[i], a.z[i]}, V3{a.dx[i], a.dy[i], a.dz[i]}, c, o, ps,
+                                         step, a.kind);
   } else {
     // A state outside the world has no volume to step in and stays outside
     // (step 0); a malformed state is left as it is, with step NaN.
@@ -123,7 +124,7 @@ __global__ __launch_bounds__(kNavBlock) void StepKernel(const NavDev g, const St
   a.step[i] = step;
 }
 
-template <int D>
+template <int D, uint32_t M>
 __global__ __launch_bounds__(kNavBlock) void SafetyKernel(const NavDev g, const double *__restrict__ x,
                                                           const double *__restrict__ y, const double *__restrict__ z,
                                                           uint64_t n, const vg_nav_states s, double *__restrict__ out) {
@@ -131,7 +132,7 @@ __global__ __launch_bounds__(kNavBlock) void SafetyKernel(const NavDev g, const 
   if (i >= n) return;
   DPath<D> c;
   LoadState(s, i, c);
-  out[i] = ValidState(g, c) ? GetSafety(g, V3{x[i], y[i], z[i]}, c) : (c.level == 0 ? 0. : __builtin_nan(""));
+  out[i] = ValidState(g, c) ? GetSafety<D, M>(g, V3{x[i], y[i], z[i]}, c) : (c.level == 0 ? 0. : __builtin_nan(""));
 }
 
 struct TraceArgs {
@@ -146,7 +147,7 @@ struct TraceArgs {
   int32_t max_steps;
 };
 
-template <int D>
+template <int D, uint32_t M>
 __global__ __launch_bounds__(kNavBlock) void TraceKernel(const NavDev g, const TraceArgs a) {
   const uint64_t i = (uint64_t)blockIdx.x * kNavBlock + threadIdx.x;
   if (i >= a.n) return;
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(kNavBlock) void TraceKernel(const NavDev g, const T
   double dist;
   uint64_t h;
   int nexact;
-  const int k = TraceOne<D>(g, p, d, a.pstep, a.kind, a.max_steps, dist, h, nexact);
+  const int k = TraceOne<D, false, M>(g, p, d, a.pstep, a.kind, a.max_steps, dist, h, nexact);
   a.nsteps[i] = k;
   a.fx[i] = p.x;
   a.fy[i] = p.y;
@@ -185,10 +186,7 @@ unsigned Blocks(uint64_t n) { return (unsigned)((n + kNavBlock - 1) / kNavBlock)
 int NavLocate(const vg_geometry *g, const double *x, const double *y, const double *z, uint64_t n,
               const vg_nav_states &s, hipStream_t st) {
   if (n == 0) return VG_OK;
-  if (g->max_depth <= 8)
-    hipLaunchKernelGGL(LocateKernel<8>, dim3(Blocks(n)), dim3(kNavBlock), 0, st, Dev(g), x, y, z, n, s);
-  else
-    hipLaunchKernelGGL(LocateKernel<16>, dim3(Blocks(n)), dim3(kNavBlock), 0, st, Dev(g), x, y, z, n, s);
+  VGA_NAV_VARIANTS(g, LocateKernel, dim3(Blocks(n)), dim3(kNavBlock), 0, st, Dev(g), x, y, z, n, s);
   return CheckHip(hipGetLastError(), "locate launch");
 }
 
@@ -197,20 +195,14 @@ int NavStep(const vg_geometry *g, int kind, const double *x, const double *y, co
             const vg_nav_states &nxt, double *step, hipStream_t st) {
   if (n == 0) return VG_OK;
   StepArgs a{x, y, z, dx, dy, dz, pstep, step, n, kind, 0};
-  if (g->max_depth <= 8)
-    hipLaunchKernelGGL(StepKernel<8>, dim3(Blocks(n)), dim3(kNavBlock), 0, st, Dev(g), a, cur, nxt);
-  else
-    hipLaunchKernelGGL(StepKernel<16>, dim3(Blocks(n)), dim3(kNavBlock), 0, st, Dev(g), a, cur, nxt);
+  VGA_NAV_VARIANTS(g, StepKernel, dim3(Blocks(n)), dim3(kNavBlock), 0, st, Dev(g), a, cur, nxt);
   return CheckHip(hipGetLastError(), "step launch");
 }
 
 int NavSafety(const vg_geometry *g, const double *x, const double *y, const double *z, uint64_t n,
               const vg_nav_states &s, double *out, hipStream_t st) {
   if (n == 0) return VG_OK;
-  if (g->max_depth <= 8)
-    hipLaunchKernelGGL(SafetyKernel<8>, dim3(Blocks(n)), dim3(kNavBlock), 0, st, Dev(g), x, y, z, n, s, out);
-  else
-    hipLaunchKernelGGL(SafetyKernel<16>, dim3(Blocks(n)), dim3(kNavBlock), 0, st, Dev(g), x, y, z, n, s, out);
+  VGA_NAV_VARIANTS(g, SafetyKernel, dim3(Blocks(n)), dim3(kNavBlock), 0, st, Dev(g), x, y, z, n, s, out);
   return CheckHip(hipGetLastError(), "safety launch");
 }
 
@@ -219,10 +211,7 @@ int NavTrace(const vg_geometry *g, int kind, const double *x, const double *y, c
              double *fy, double *fz, double *travelled, uint64_t *hash, int32_t *exact, hipStream_t st) {
   if (n == 0) return VG_OK;
   TraceArgs a{x, y, z, dx, dy, dz, nsteps, fx, fy, fz, travelled, hash, exact, n, pstep, kind, max_steps};
-  if (g->max_depth <= 8)
-    hipLaunchKernelGGL(TraceKernel<8>, dim3(Blocks(n)), dim3(kNavBlock), 0, st, Dev(g), a);
-  else
-    hipLaunchKernelGGL(TraceKernel<16>, dim3(Blocks(n)), dim3(kNavBlock), 0, st, Dev(g), a);
+  VGA_NAV_VARIANTS(g, TraceKernel, dim3(Blocks(n)), dim3(kNavBlock), 0, st, Dev(g), a);
   return CheckHip(hipGetLastError(), "trace launch");
 }
 

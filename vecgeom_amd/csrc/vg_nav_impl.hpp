@@ -55,81 +55,123 @@ struct NavDev {
   }
 };
 
+// Shape sets of specialised navigators (bit t: shape type t).
+constexpr uint32_t kAllShapes = (1u << VG_BOX) | (1u << VG_TUBE) | (1u << VG_ORB) | (1u << VG_SPHERE) |
+                                (1u << VG_CONE) | (1u << VG_TRD) | (1u << VG_TRAPEZOID) |
+                                (1u << VG_PARALLELEPIPED) | (1u << VG_PARABOLOID) | (1u << VG_TORUS) |
+                                (1u << VG_POLYCONE) | (1u << VG_POLYHEDRON);
+// Box, Tube, Orb and Trd: the fused trace kernel needs 145 VGPRs with these
+// alone (three waves per SIMD) against 230 with every shape (two).
+constexpr uint32_t kBasicShapes = (1u << VG_BOX) | (1u << VG_TUBE) | (1u << VG_ORB) | (1u << VG_TRD);
+
 // ------------------------------------------------------------ shape switch
-#define VGA_SHAPE_CASES(TYPE, BODY)            \
-  switch (TYPE) {                              \
-    case VG_BOX: {                             \
-      using S = BoxS;                          \
-      BODY;                                    \
-    }                                          \
-    case VG_TUBE: {                            \
-      using S = TubeS;                         \
-      BODY;                                    \
-    }                                          \
-    case VG_ORB: {                             \
-      using S = OrbS;                          \
-      BODY;                                    \
-    }                                          \
-    case VG_TRD: {                             \
-      using S = TrdS;                          \
-      BODY;                                    \
-    }                                          \
-    case VG_PARALLELEPIPED: {                  \
-      using S = ParaS;                         \
-      BODY;                                    \
-    }                                          \
-    case VG_PARABOLOID: {                      \
-      using S = ParabS;                        \
-      BODY;                                    \
-    }                                          \
-    case VG_TRAPEZOID: {                       \
-      using S = TrapS;                         \
-      BODY;                                    \
-    }                                          \
-    case VG_SPHERE: {                          \
-      using S = SphereS;                       \
-      BODY;                                    \
-    }                                          \
-    case VG_CONE: {                            \
-      using S = ConeS;                         \
-      BODY;                                    \
-    }                                          \
-    case VG_POLYCONE: {                        \
-      using S = PolyconeS;                     \
-      BODY;                                    \
-    }                                          \
-    case VG_TORUS: {                           \
-      using S = TorusS;                        \
-      BODY;                                    \
-    }                                          \
-    case VG_POLYHEDRON: {                      \
-      using S = PolyhedronS;                   \
-      BODY;                                    \
-    }                                          \
+// Shape switch over the compile-time shape set M (bit t set: shape type t
+// may occur).  A navigator specialised for the shapes of one geometry
+// carries only their code, which keeps its register count down.
+#define VGA_SHAPE_CASES(TYPE, BODY)                   \
+  switch (TYPE) {                                     \
+    case VG_BOX:                                      \
+      if constexpr ((M >> VG_BOX) & 1u) {             \
+        using S = BoxS;                               \
+        BODY;                                         \
+      }                                               \
+      break;                                          \
+    case VG_TUBE:                                     \
+      if constexpr ((M >> VG_TUBE) & 1u) {            \
+        using S = TubeS;                              \
+        BODY;                                         \
+      }                                               \
+      break;                                          \
+    case VG_ORB:                                      \
+      if constexpr ((M >> VG_ORB) & 1u) {             \
+        using S = OrbS;                               \
+        BODY;                                         \
+      }                                               \
+      break;                                          \
+    case VG_TRD:                                      \
+      if constexpr ((M >> VG_TRD) & 1u) {             \
+        using S = TrdS;                               \
+        BODY;                                         \
+      }                                               \
+      break;                                          \
+    case VG_PARALLELEPIPED:                           \
+      if constexpr ((M >> VG_PARALLELEPIPED) & 1u) {  \
+        using S = ParaS;                              \
+        BODY;                                         \
+      }                                               \
+      break;                                          \
+    case VG_PARABOLOID:                               \
+      if constexpr ((M >> VG_PARABOLOID) & 1u) {      \
+        using S = ParabS;                             \
+        BODY;                                         \
+      }                                               \
+      break;                                          \
+    case VG_TRAPEZOID:                                \
+      if constexpr ((M >> VG_TRAPEZOID) & 1u) {       \
+        using S = TrapS;                              \
+        BODY;                                         \
+      }                                               \
+      break;                                          \
+    case VG_SPHERE:                                   \
+      if constexpr ((M >> VG_SPHERE) & 1u) {          \
+        using S = SphereS;                            \
+        BODY;                                         \
+      }                                               \
+      break;                                          \
+    case VG_CONE:                                     \
+      if constexpr ((M >> VG_CONE) & 1u) {            \
+        using S = ConeS;                              \
+        BODY;                                         \
+      }                                               \
+      break;                                          \
+    case VG_POLYCONE:                                 \
+      if constexpr ((M >> VG_POLYCONE) & 1u) {        \
+        using S = PolyconeS;                          \
+        BODY;                                         \
+      }                                               \
+      break;                                          \
+    case VG_TORUS:                                    \
+      if constexpr ((M >> VG_TORUS) & 1u) {           \
+        using S = TorusS;                             \
+        BODY;                                         \
+      }                                               \
+      break;                                          \
+    case VG_POLYHEDRON:                               \
+      if constexpr ((M >> VG_POLYHEDRON) & 1u) {      \
+        using S = PolyhedronS;                        \
+        BODY;                                         \
+      }                                               \
+      break;                                          \
   }
 
 #define VGA_P(blk) (ParamView<S>::Get(blk))
 
+template <uint32_t M = kAllShapes>
 VGA_NAV bool DContains(int type, const double *blk, V3 p) {
   VGA_SHAPE_CASES(type, return S::Contains(VGA_P(blk), p));
   return false;
 }
+template <uint32_t M = kAllShapes>
 VGA_NAV int DInside(int type, const double *blk, V3 p) {
   VGA_SHAPE_CASES(type, return S::Inside(VGA_P(blk), p));
   return kOutside;
 }
+template <uint32_t M = kAllShapes>
 VGA_NAV double DDistIn(int type, const double *blk, V3 p, V3 d, double sm) {
   VGA_SHAPE_CASES(type, return S::DistanceToIn(VGA_P(blk), p, d, sm));
   return kInfinity;
 }
+template <uint32_t M = kAllShapes>
 VGA_NAV double DDistOut(int type, const double *blk, V3 p, V3 d, double sm) {
   VGA_SHAPE_CASES(type, return S::DistanceToOut(VGA_P(blk), p, d, sm));
   return kInfinity;
 }
+template <uint32_t M = kAllShapes>
 VGA_NAV double DSafeIn(int type, const double *blk, V3 p) {
   VGA_SHAPE_CASES(type, return S::SafetyToIn(VGA_P(blk), p));
   return kInfinity;
 }
+template <uint32_t M = kAllShapes>
 VGA_NAV double DSafeOut(int type, const double *blk, V3 p) {
   VGA_SHAPE_CASES(type, return S::SafetyToOut(VGA_P(blk), p));
   return kInfinity;
@@ -258,29 +300,31 @@ VGA_NAV Xf TopMatrix(const NavDev &g, const DPath<D> &s) {
 
 // Contains of a placed volume on a point of its mother frame; the local
 // point is returned through `local`.
+template <uint32_t M = kAllShapes>
 VGA_NAV bool PlacedContains(const NavDev &g, int pv, V3 p, V3 &local) {
   const PlacedRec &r = g.Pv(pv);
   Xf x;
   LoadXf(r, x);
   local = XfPointRt(x, r.xfkind, p);
-  return DContains(r.type, g.params + r.param_off, local);
+  return DContains<M>(r.type, g.params + r.param_off, local);
 }
 
+template <uint32_t M = kAllShapes>
 VGA_NAV double PlacedDistIn(const NavDev &g, int pv, V3 p, V3 d, double sm) {
   const PlacedRec &r = g.Pv(pv);
   Xf x;
   LoadXf(r, x);
-  return DDistIn(r.type, g.params + r.param_off, XfPointRt(x, r.xfkind, p), XfDirRt(x, r.xfkind, d), sm);
+  return DDistIn<M>(r.type, g.params + r.param_off, XfPointRt(x, r.xfkind, p), XfDirRt(x, r.xfkind, d), sm);
 }
 
 // ----------------------------------------------------------- locate family
-template <int D>
+template <int D, uint32_t M = kAllShapes>
 VGA_NAV int LocatePoint(const NavDev &g, int vol, V3 point, DPath<D> &path, bool top) {
   int cand = vol;
   V3 tmp = point;
   if (top) {
     const PlacedRec &r = g.Pv(vol);
-    cand = DContains(r.type, g.params + r.param_off, point) ? vol : -1;
+    cand = DContains<M>(r.type, g.params + r.param_off, point) ? vol : -1;
   }
   if (cand >= 0) {
     path.Push(cand);
@@ -292,7 +336,7 @@ VGA_NAV int LocatePoint(const NavDev &g, int vol, V3 point, DPath<D> &path, bool
       for (int k = db; k < de; ++k) {
         const int nx = g.Daughter(k);
         V3 tp;
-        if (PlacedContains(g, nx, tmp, tp)) {
+        if (PlacedContains<M>(g, nx, tmp, tp)) {
           path.Push(nx);
           tmp = tp;
           cand = nx;
@@ -305,7 +349,7 @@ VGA_NAV int LocatePoint(const NavDev &g, int vol, V3 point, DPath<D> &path, bool
   return cand;
 }
 
-template <int D>
+template <int D, uint32_t M = kAllShapes>
 VGA_NAV int LocatePointExcl(const NavDev &g, int vol, int excluded, V3 point, DPath<D> &path) {
   int cand = vol;
   V3 tmp = point;
@@ -318,7 +362,7 @@ VGA_NAV int LocatePointExcl(const NavDev &g, int vol, int excluded, V3 point, DP
     for (int k = de - 1; k >= db; --k) {
       const int nx = g.Daughter(k);
       V3 tp;
-      if (nx != excluded && PlacedContains(g, nx, tmp, tp)) {
+      if (nx != excluded && PlacedContains<M>(g, nx, tmp, tp)) {
         path.Push(nx);
         tmp = tp;
         cand = nx;
@@ -330,14 +374,14 @@ VGA_NAV int LocatePointExcl(const NavDev &g, int vol, int excluded, V3 point, DP
   return cand;
 }
 
-template <int D>
+template <int D, uint32_t M = kAllShapes>
 VGA_NAV int Relocate(const NavDev &g, V3 localpoint, DPath<D> &path) {
   int mother = path.Top();
   if (mother >= 0) {
     V3 tmp = localpoint;
     while (mother >= 0) {
       const PlacedRec &r = g.Pv(mother);
-      if (DContains(r.type, g.params + r.param_off, tmp)) break;
+      if (DContains<M>(r.type, g.params + r.param_off, tmp)) break;
       path.Pop();
       Xf x;
       LoadXf(r, x);
@@ -346,7 +390,7 @@ VGA_NAV int Relocate(const NavDev &g, V3 localpoint, DPath<D> &path) {
     }
     if (mother >= 0) {
       path.Pop();
-      return LocatePointExcl(g, mother, mother, tmp, path);
+      return LocatePointExcl<D, M>(g, mother, mother, tmp, path);
     }
   }
   return mother;
@@ -401,7 +445,7 @@ VGA_NAV Ray MakeRay(V3 lp, V3 ld) {
 // holds the mother's DistanceToOut (kInfinity if negative).  Returns 1 if
 // the exact sorted-order walk was needed.  ExactOnly skips the order-free
 // walk (used by the host-side checks of the exact walk).
-template <bool ExactOnly = false>
+template <bool ExactOnly = false, uint32_t M = kAllShapes>
 VGA_NAV int AbboxCandidates(const NavDev &g, int db, int de, V3 lp, V3 ld, double &step, int &next) {
   const Ray ray = MakeRay(lp, ld);
   const double step0 = step;
@@ -417,11 +461,11 @@ VGA_NAV int AbboxCandidates(const NavDev &g, int db, int de, V3 lp, V3 ld, doubl
     LoadXf(r, x);
     const V3 tp = XfPointRt(x, r.xfkind, lp);
     const double *blk = g.params + r.param_off;
-    if (b < 0.f && DContains(r.type, blk, tp)) {
+    if (b < 0.f && DContains<M>(r.type, blk, tp)) {
       exact = true;
       break;
     }
-    const double dd = DDistIn(r.type, blk, tp, XfDirRt(x, r.xfkind, ld), step0);
+    const double dd = DDistIn<M>(r.type, blk, tp, XfDirRt(x, r.xfkind, ld), step0);
     if (dd < 0. || dd < (double)b) {
       exact = true;
       break;
@@ -466,13 +510,13 @@ VGA_NAV int AbboxCandidates(const NavDev &g, int db, int de, V3 lp, V3 ld, doubl
     const int cand = g.Daughter(db + kk);
     if (hd < 0) {
       V3 tp;
-      if (PlacedContains(g, cand, lp, tp)) {
+      if (PlacedContains<M>(g, cand, lp, tp)) {
         step = 0;
         next = kk;
         break;
       }
     }
-    const double dd = PlacedDistIn(g, cand, lp, ld, step);
+    const double dd = PlacedDistIn<M>(g, cand, lp, ld, step);
     if (dd < step) {
       next = kk;
       step = dd;
@@ -483,7 +527,7 @@ VGA_NAV int AbboxCandidates(const NavDev &g, int db, int de, V3 lp, V3 ld, doubl
 
 // FindNextBoundaryAndStep of both navigators (kind 0 ABBox, 1 Simple).
 // Returns 1 if the ABBox walk took the exact path.
-template <int D, bool ExactOnly = false>
+template <int D, bool ExactOnly = false, uint32_t M = kAllShapes>
 VGA_NAV int FindNextBoundaryAndStep(const NavDev &g, V3 gp, V3 gd, const DPath<D> &cur, DPath<D> &nst,
                                        double pstep, double &step, int kind) {
   const Xf m = TopMatrix(g, cur);
@@ -494,11 +538,11 @@ VGA_NAV int FindNextBoundaryAndStep(const NavDev &g, V3 gp, V3 gd, const DPath<D
   const int db = C.dbegin, de = C.dend;
   int next = -1;
   int exact = 0;
-  step = DDistOut(C.type, g.params + C.param_off, lp, ld, pstep);
+  step = DDistOut<M>(C.type, g.params + C.param_off, lp, ld, pstep);
   nst = cur;
   if (kind == VG_NAV_ABBOX) {
     if (step < 0.) step = kInfinity;
-    if (de > db) exact = AbboxCandidates<ExactOnly>(g, db, de, lp, ld, step, next);
+    if (de > db) exact = AbboxCandidates<ExactOnly, M>(g, db, de, lp, ld, step, next);
     if (step > pstep) {
       step = pstep;
       nst.boundary = 0;
@@ -514,7 +558,7 @@ VGA_NAV int FindNextBoundaryAndStep(const NavDev &g, V3 gp, V3 gd, const DPath<D
   } else {
     if (step <= 0.) step = kInfinity;
     for (int k = db; k < de; ++k) {
-      const double dd = PlacedDistIn(g, g.Daughter(k), lp, ld, step);
+      const double dd = PlacedDistIn<M>(g, g.Daughter(k), lp, ld, step);
       const bool valid = dd < step && !__builtin_isinf(dd);
       next = valid ? k - db : next;
       step = valid ? dd : step;
@@ -539,27 +583,27 @@ VGA_NAV int FindNextBoundaryAndStep(const NavDev &g, V3 gp, V3 gd, const DPath<D
     const int nv = g.Daughter(db + next);
     Xf x;
     LoadXf(g.Pv(nv), x);
-    LocatePoint(g, nv, ApplyGeneric(x, np), nst, false);
+    LocatePoint<D, M>(g, nv, ApplyGeneric(x, np), nst, false);
     return exact;
   }
-  Relocate(g, np, nst);
+  Relocate<D, M>(g, np, nst);
   if (kind == VG_NAV_ABBOX) {
     while (nst.Top() == cv) nst.Pop();
   }
   return exact;
 }
 
-template <int D>
+template <int D, uint32_t M = kAllShapes>
 VGA_NAV double GetSafety(const NavDev &g, V3 gp, const DPath<D> &cur) {
   const Xf m = TopMatrix(g, cur);
   const V3 lp = ApplyGeneric(m, gp);
   const PlacedRec &C = g.Pv(cur.Top());
-  double safety = DSafeOut(C.type, g.params + C.param_off, lp);
+  double safety = DSafeOut<M>(C.type, g.params + C.param_off, lp);
   for (int k = C.dbegin; k < C.dend; ++k) {
     const PlacedRec &r = g.Pv(g.Daughter(k));
     Xf x;
     LoadXf(r, x);
-    const double t = DSafeIn(r.type, g.params + r.param_off, XfPointRt(x, r.xfkind, lp));
+    const double t = DSafeIn<M>(r.type, g.params + r.param_off, XfPointRt(x, r.xfkind, lp));
     safety = Min(safety, t);
   }
   return safety;
@@ -570,19 +614,19 @@ VGA_NAV double GetSafety(const NavDev &g, V3 gp, const DPath<D> &cur) {
 // state leaves the world or max_steps steps were taken.  Returns the number
 // of steps; p ends at the final point.  The hash folds
 // (new top id + 1) * 2 + boundary after every step.
-template <int D, bool ExactOnly = false>
+template <int D, bool ExactOnly = false, uint32_t M = kAllShapes>
 VGA_NAV int TraceOne(const NavDev &g, V3 &p, V3 d, double pstep, int kind, int max_steps, double &dist, uint64_t &h,
                      int &nexact) {
   DPath<D> cur, nst;
   cur.Clear();
-  LocatePoint(g, g.world, p, cur, true);
+  LocatePoint<D, M>(g, g.world, p, cur, true);
   int k = 0;
   nexact = 0;
   dist = 0;
   h = 1469598103934665603ULL;
   while (cur.level > 0 && k < max_steps) {
     double step;
-    nexact += FindNextBoundaryAndStep<D, ExactOnly>(g, p, d, cur, nst, pstep, step, kind);
+    nexact += FindNextBoundaryAndStep<D, ExactOnly, M>(g, p, d, cur, nst, pstep, step, kind);
     dist += step;
     const double s = step + 1E-6;
     p = V3{p.x + d.x * s, p.y + d.y * s, p.z + d.z * s};
@@ -596,3 +640,22 @@ VGA_NAV int TraceOne(const NavDev &g, V3 &p, V3 d, double pstep, int kind, int m
 
 }  // namespace nav
 }  // namespace vga
+
+// Launches KERNEL<D, M> with the path depth D (8 or 16) that holds the
+// geometry and the shape set M: the basic shapes only when the geometry uses
+// nothing else, all shapes otherwise.
+#define VGA_NAV_VARIANTS(G, KERNEL, ...)                                          \
+  do {                                                                            \
+    const bool light_ = ((G)->type_mask & ~::vga::nav::kBasicShapes) == 0;        \
+    if ((G)->max_depth <= 8) {                                                    \
+      if (light_)                                                                 \
+        hipLaunchKernelGGL((KERNEL<8, ::vga::nav::kBasicShapes>), __VA_ARGS__);   \
+      else                                                                        \
+        hipLaunchKernelGGL((KERNEL<8, ::vga::nav::kAllShapes>), __VA_ARGS__);     \
+    } else {                                                                      \
+      if (light_)                                                                 \
+        hipLaunchKernelGGL((KERNEL<16, ::vga::nav::kBasicShapes>), __VA_ARGS__);  \
+      else                                                                        \
+        hipLaunchKernelGGL((KERNEL<16, ::vga::nav::kAllShapes>), __VA_ARGS__);    \
+    }                                                                             \
+  } while (0)

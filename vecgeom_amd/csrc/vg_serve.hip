@@ -81,7 +81,7 @@ __device__ __forceinline__ int FindSeg(const Seg *segs, int nseg, uint64_t tile)
 }
 
 // Locates new tracks (slots [offset, offset + n)) from the world.
-template <int D>
+template <int D, uint32_t M>
 __global__ __launch_bounds__(kServeBlock) void ServeLocateKernel(const NavDev g, const Arena a, uint64_t offset,
                                                                  uint64_t n) {
   const uint64_t i = (uint64_t)blockIdx.x * kServeBlock + threadIdx.x;
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kServeBlock) void ServeLocateKernel(const NavDev g,
   const uint64_t s = offset + i;
   DPath<D> p;
   p.Clear();
-  LocatePoint(g, g.world, V3{a.x[s], a.y[s], a.z[s]}, p, true);
+  LocatePoint<D, M>(g, g.world, V3{a.x[s], a.y[s], a.z[s]}, p, true);
   a.level[s] = p.level;
   a.boundary[s] = 0;
   for (int l = 0; l < a.depth; ++l) a.path[(uint64_t)l * a.cap + s] = l < D ? p.Get(l) : -1;
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(kServeBlock) void ServeLocateKernel(const NavDev g,
 // One step of every track of every segment; a track outside the world
 // (level 0) stays where it is with step 0.  Tiles of kServeBlock tracks are
 // numbered across segments; each block finds its segment by binary search.
-template <int D>
+template <int D, uint32_t M>
 __global__ __launch_bounds__(kServeBlock) void ServeStepKernel(const NavDev g, const Arena a,
                                                                const Seg *__restrict__ segs, int nseg, uint64_t ntiles,
                                                                double pstep, int kind) {
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kServeBlock) void ServeStepKernel(const NavDev g, c
   V3 p{a.x[s], a.y[s], a.z[s]};
   const V3 d{a.dx[s], a.dy[s], a.dz[s]};
   double step;
-  FindNextBoundaryAndStep<D>(g, p, d, cur, nst, pstep, step, kind);
+  FindNextBoundaryAndStep<D, false, M>(g, p, d, cur, nst, pstep, step, kind);
   const double sp = step + 1E-6;
   p = V3{p.x + d.x * sp, p.y + d.y * sp, p.z + d.z * sp};
   a.x[s] = p.x;
@@ -232,12 +232,8 @@ void StartLaunch(vg_server *s, int k) {
     rc = CheckHip(hipMemcpyAsync(s->d_segs[k], s->h_segs[k], nseg * sizeof(Seg), hipMemcpyHostToDevice, s->stream[k]),
                   "serve segment upload");
     if (!rc) {
-      if (s->g->max_depth <= 8)
-        hipLaunchKernelGGL((ServeStepKernel<8>), dim3((unsigned)tiles), dim3(kServeBlock), 0, s->stream[k], s->dev,
-                           s->a, s->d_segs[k], nseg, tiles, s->pstep, s->navigator);
-      else
-        hipLaunchKernelGGL((ServeStepKernel<16>), dim3((unsigned)tiles), dim3(kServeBlock), 0, s->stream[k], s->dev,
-                           s->a, s->d_segs[k], nseg, tiles, s->pstep, s->navigator);
+      VGA_NAV_VARIANTS(s->g, ServeStepKernel, dim3((unsigned)tiles), dim3(kServeBlock), 0, s->stream[k], s->dev, s->a,
+                       s->d_segs[k], nseg, tiles, s->pstep, s->navigator);
       rc = CheckHip(hipGetLastError(), "serve step launch");
     }
   }
@@ -367,11 +363,7 @@ int Open(vg_server *s, uint64_t n, const double *const src[6], int64_t *basket) 
     rc = CheckHip(hipMemcpyAsync(dst[i] + off, src[i], n * 8, hipMemcpyHostToDevice, s->ustream), "basket upload");
   if (!rc && n) {
     const unsigned blocks = (unsigned)((n + kServeBlock - 1) / kServeBlock);
-    if (s->g->max_depth <= 8)
-      hipLaunchKernelGGL((ServeLocateKernel<8>), dim3(blocks), dim3(kServeBlock), 0, s->ustream, s->dev, s->a, off, n);
-    else
-      hipLaunchKernelGGL((ServeLocateKernel<16>), dim3(blocks), dim3(kServeBlock), 0, s->ustream, s->dev, s->a, off,
-                         n);
+    VGA_NAV_VARIANTS(s->g, ServeLocateKernel, dim3(blocks), dim3(kServeBlock), 0, s->ustream, s->dev, s->a, off, n);
     rc = CheckHip(hipGetLastError(), "basket locate");
   }
   if (!rc) rc = CheckHip(hipStreamSynchronize(s->ustream), "basket open sync");

@@ -191,6 +191,64 @@ __global__ __launch_bounds__(kBlock) void ShapeQueryKernel(const typename S::P P
   }
 }
 
+template <class S, class = void>
+struct HasQuick {
+  static constexpr bool value = false;
+};
+template <class S>
+struct HasQuick<S, decltype((void)S::kHasQuick)> {
+  static constexpr bool value = S::kHasQuick;
+};
+
+// DistanceToIn with lane compaction, for shapes whose DistanceToIn starts
+// with a cheap exact rejection (a bounding volume the ray misses).  Phase 1:
+// every lane runs the rejection and writes kInfinity for rejected rays; the
+// others append their query to a workgroup list in LDS.  Phase 2: the first
+// `count` lanes run the full DistanceToIn on the listed queries, so the
+// expensive part runs on dense waves instead of waves where most lanes idle.
+template <class S, int XK, bool Stage>
+__global__ __launch_bounds__(kBlock) void CompactDistInKernel(const typename S::P Pg, const Xf X, const QArgs a,
+                                                              int stage_len) {
+  typename S::P P = Pg;
+  if constexpr (Stage) {
+    extern __shared__ double stage[];
+    for (int k = threadIdx.x; k < stage_len; k += kBlock) stage[k] = Pg.blk[k];
+    P = typename S::P{stage};
+  }
+  __shared__ int count;
+  __shared__ int lane[kBlock];
+  __shared__ double qp[3][kBlock], qd[3][kBlock], qsm[kBlock];
+  if (threadIdx.x == 0) count = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kBlock;
+  const uint64_t i = base + threadIdx.x;
+  if (i < a.n) {
+    const V3 p = XfPoint<XK>(X, V3{a.x[i], a.y[i], a.z[i]});
+    const V3 d = XfDir<XK>(X, V3{a.dx[i], a.dy[i], a.dz[i]});
+    const double sm = a.sm ? a.sm[i] : kInfinity;
+    double r;
+    if (S::DistanceToInQuick(P, p, d, sm, r)) {
+      reinterpret_cast<double *>(a.out)[i] = r;
+    } else {
+      const int k = atomicAdd(&count, 1);
+      lane[k] = threadIdx.x;
+      qp[0][k] = p.x;
+      qp[1][k] = p.y;
+      qp[2][k] = p.z;
+      qd[0][k] = d.x;
+      qd[1][k] = d.y;
+      qd[2][k] = d.z;
+      qsm[k] = sm;
+    }
+  }
+  __syncthreads();
+  const int k = threadIdx.x;
+  if (k < count) {
+    reinterpret_cast<double *>(a.out)[base + lane[k]] =
+        S::DistanceToIn(P, V3{qp[0][k], qp[1][k], qp[2][k]}, V3{qd[0][k], qd[1][k], qd[2][k]}, qsm[k]);
+  }
+}
+
 // Plain FP64 form (the benchmark denominator): one thread per query, the
 // placement record and shape parameters read from global memory through the
 // placed-volume id (as the reference's device virtual call chases its object
@@ -234,6 +292,18 @@ int LaunchQ(const typename S::P &P, const Xf &X, const QArgs &a, int stage_len, 
   bool al = Aligned16(a.x) && Aligned16(a.y) && Aligned16(a.z) && Aligned16(a.out);
   if (NeedsDir(Q)) al = al && Aligned16(a.dx) && Aligned16(a.dy) && Aligned16(a.dz);
   if (NeedsStep(Q) && a.sm) al = al && Aligned16(a.sm);
+  if constexpr (Q == kQDistIn && HasQuick<S>::value) {
+    const uint64_t blocks = (a.n + kBlock - 1) / kBlock;
+    if constexpr (ParamView<S>::kIndirect) {
+      if (stage_len > 0 && stage_len <= kMaxStageDoubles) {
+        hipLaunchKernelGGL((CompactDistInKernel<S, XK, true>), dim3((unsigned)blocks), dim3(kBlock),
+                           stage_len * sizeof(double), s, P, X, a, stage_len);
+        return CheckHip(hipGetLastError(), "shape query launch");
+      }
+    }
+    hipLaunchKernelGGL((CompactDistInKernel<S, XK, false>), dim3((unsigned)blocks), dim3(kBlock), 0, s, P, X, a, 0);
+    return CheckHip(hipGetLastError(), "shape query launch");
+  }
   if constexpr (ParamView<S>::kIndirect) {
     if (stage_len > 0 && stage_len <= kMaxStageDoubles) {
       const uint64_t blocks = (a.n + kBlock - 1) / kBlock;
