@@ -1,4 +1,5 @@
-"""Known answers of the reference's unit tests (TestBox.cpp, TestTube.cpp, TestPolyhedra.cpp),
+"""Known answers of the reference's unit tests (TestBox.cpp, TestTube.cpp, TestPolycone.cpp,
+TestPolyhedra.cpp),
 transcribed as data in tests/golden/known_answers.json, checked on the
 oracle here and on the GPU in test_gpu_shapes.py."""
 import json

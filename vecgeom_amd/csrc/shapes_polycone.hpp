@@ -99,25 +99,6 @@ struct PolyconeS {
   // The reference's polycone Inside reports inside or outside only.
   static VGA_HD int Inside(const P &q, V3 p) { return Contains(q, p) ? kInside : kOutside; }
 
-  // True (and out = kInfinity) when no section DistanceToIn would visit
-  // comes near the ray: every visited section then gives kInfinity.
-  static constexpr bool kHasQuick = true;
-  static VGA_HD bool DistanceToInQuick(const P &q, V3 p, V3 v, double, double &out) {
-    const int n = NSec(q);
-    int increment = (v.z > 0) ? 1 : -1;
-    if (Abs(v.z) < kTolerance) increment = 0;
-    int index = SectionIndex(q, p.z);
-    if (index == -1) index = 0;
-    if (index == -2) index = n - 1;
-    do {
-      if (RayNearSection(q, index, p, v)) return false;
-      if (!increment) break;
-      index += increment;
-    } while (index >= 0 && index < n);
-    out = kInfinity;
-    return true;
-  }
-
   static VGA_HD double DistanceToIn(const P &q, V3 p, V3 v, double stepMax) {
     const int n = NSec(q);
     double distance = kInfinity;

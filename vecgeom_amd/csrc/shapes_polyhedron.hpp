@@ -183,42 +183,49 @@ struct PolyhedronS {
     }
     return valid;
   }
-  // Quadrilaterals::DistanceToIn<kScalar, behind>, Quadrilaterals.h:302-350
+  // Quadrilaterals::DistanceToIn<kScalar, behind>, Quadrilaterals.h:302-350:
+  // the loop ends at the first quadrilateral whose hit leaves best finite.
+  // Written without early exits inside an iteration -- every lane evaluates
+  // every test of the quadrilateral with the same arithmetic and masks the
+  // outcome -- so that the lanes of a wave stay converged; `done` replays the
+  // reference's break.
   template <bool Behind>
   static VGA_HD double QDistanceToIn(const Quads &s, V3 p, V3 d) {
     double best = kInfinity;
-    for (int i = 0; i < s.n; ++i) {
-      const V3 nn = s.N(i);
-      double distance = Dot(p, nn) + s.D(i);
-      if (!((Behind ? -distance : distance) >= 0)) continue;
+    bool done = false;
+    for (int i = 0; i < s.n && !done; ++i) {
+      const double *R = s.R(i);
+      const V3 nn{R[0], R[1], R[2]};
+      const double dist = Dot(p, nn) + R[3];
       const double proj = Dot(d, nn);
-      if (!((Behind ? proj : -proj) >= 0)) continue;
-      distance /= -proj;
-      const V3 x = p + d * distance;
-      if (RayHits(s, i, x)) best = distance;
-      if (best < kInfinity) break;
+      const double t = dist / -proj;
+      const V3 x = p + d * t;
+      bool valid = ((Behind ? -dist : dist) >= 0) & ((Behind ? proj : -proj) >= 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) valid &= x.x * R[4 + 4 * j] + x.y * R[5 + 4 * j] + x.z * R[6 + 4 * j] + R[7 + 4 * j] >= 0;
+      best = valid ? t : best;
+      done = best < kInfinity;
     }
     return best;
   }
-  // Quadrilaterals::DistanceToOut<kScalar>, Quadrilaterals.h:435-490
+  // Quadrilaterals::DistanceToOut<kScalar>, Quadrilaterals.h:435-490, branch
+  // free within an iteration as QDistanceToIn.
   static VGA_HD double QDistanceToOut(const Quads &s, V3 p, V3 d, double zMin, double zMax) {
     double best = kInfinity;
+    const bool degenerate = zMin == zMax;
     for (int i = 0; i < s.n; ++i) {
       const V3 nn = s.N(i);
-      double t = Dot(p, nn) + s.D(i);
-      if (!(t < 0)) continue;
+      const double t0 = Dot(p, nn) + s.D(i);
       const double proj = Dot(d, nn);
-      if (!(proj >= 0)) continue;
-      t /= -proj;
-      if (!(t < best)) continue;
-      bool valid;
-      if (zMin == zMax) {
-        valid = RayHits(s, i, p + t * d);
+      const double t = t0 / -proj;
+      bool valid = (t0 < 0) & (proj >= 0) & (t < best);
+      if (degenerate) {
+        valid = valid && RayHits(s, i, p + t * d);
       } else {
         const double zp = p.z + t * d.z;
-        valid = zp >= zMin && zp < zMax;
+        valid &= (zp >= zMin) & (zp < zMax);
       }
-      if (valid) best = t;
+      best = valid ? t : best;
     }
     return best;
   }
@@ -386,16 +393,6 @@ struct PolyhedronS {
     const double tSq = t * t;
     if (tSq >= distSq) return;
     if (OnEndcap(q, GetSeg(q, isFirst ? 0 : n - 1), V3{p.x, p.y, p.z + t})) distSq = tSq;
-  }
-
-  // The bounding-tube rejection at the head of DistanceToIn: true (and
-  // out = kInfinity) exactly when DistanceToIn returns kInfinity there.
-  static constexpr bool kHasQuick = true;
-  static VGA_HD bool DistanceToInQuick(const P &q, V3 p, V3 d, double stepMax, double &out) {
-    const V3 bp = BoundsPoint(q, p);
-    if (TubeS::Contains(Tube(q), bp) || TubeS::DistanceToIn(Tube(q), bp, d, stepMax) != kInfinity) return false;
-    out = kInfinity;
-    return true;
   }
 
   // ScalarDistanceToInKernel, :832-913 (local point and direction)
