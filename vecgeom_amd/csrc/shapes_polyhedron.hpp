@@ -22,9 +22,10 @@
 //   [16 ..] TubeP of the bounding tube with its treatment flags fixed as the
 //   reference's HollowTube / NonHollowTube kernels fix them
 //   segment record (10): hasInner, nOuter, nPhi, nInner, offOuter, offPhi,
-//   offInner, rLo, rHi, unused -- every quadrilateral of the segment lies in
+//   offInner, rLo, rHi, rInHi -- every quadrilateral of the segment lies in
 //   the annular slab Z(k) <= z <= Z(k+1), rLo <= rho <= rHi (rLo: the
-//   smaller inner side distance, rHi: the larger outer corner radius)
+//   smaller inner side distance, rHi: the larger corner radius), and the
+//   inner quadrilaterals within rho <= rInHi (the larger inner corner radius)
 //   quadrilateral record (33): plane normal (3) and distance, four side
 //   planes (normal 3, distance), four corners (3 each), unused
 #pragma once
@@ -89,12 +90,14 @@ struct PolyhedronS {
   // here only when its result provably cannot change the outcome, so the
   // results are those of the full walk.
 
-  // False only when no quadrilateral of segment k can yield a hit: a hit is a
+  // 0 only when no quadrilateral of segment k can yield a hit: a hit is a
   // point of one of the segment's quadrilaterals (to within rounding), and
   // those lie in the segment's annular slab, so a ray (t >= 0) that stays
   // more than kPad (1e-6 mm, far above every rounding of the tests) away from
-  // the slab gets kInfinity from every quadrilateral of the segment.
-  static VGA_HD bool RayNearSegment(const P &q, int k, V3 p, V3 v) {
+  // the slab gets kInfinity from every quadrilateral of the segment.  1 when
+  // the ray may hit the outer or phi quadrilaterals but stays more than kPad
+  // outside radius rInHi within the slab (no inner hit possible), 2 otherwise.
+  static VGA_HD int RayNearSegment(const P &q, int k, V3 p, V3 v) {
     constexpr double kPad = 1e-6;
     const double *r = SegRec(q, k);
     const double zlo = Z(q, k) - kPad, zhi = Z(q, k + 1) + kPad;
@@ -117,16 +120,18 @@ struct PolyhedronS {
     double t = a > 0 ? -b / a : t0;
     t = t < t0 ? t0 : (t > t1 ? t1 : t);
     const double x = p.x + t * v.x, y = p.y + t * v.y;
+    const double rho2 = x * x + y * y;  // smallest rho^2 within the slab
     const double rOut = r[8] + kPad;
-    if (x * x + y * y > rOut * rOut) return false;
+    if (rho2 > rOut * rOut) return 0;
     const double rIn = r[7] - kPad;
     if (rIn > 0 && t1 < kInfinity) {
       // rho^2 is convex along the ray: its maximum on [t0, t1] is at an end.
       const double x0 = p.x + t0 * v.x, y0 = p.y + t0 * v.y, x1 = p.x + t1 * v.x, y1 = p.y + t1 * v.y;
       const double r0 = x0 * x0 + y0 * y0, r1 = x1 * x1 + y1 * y1;
-      if ((r0 > r1 ? r0 : r1) < rIn * rIn) return false;
+      if ((r0 > r1 ? r0 : r1) < rIn * rIn) return 0;
     }
-    return true;
+    const double rInHi = r[9] + kPad;
+    return rho2 > rInHi * rInHi ? 1 : 2;
   }
 
   // True only when segment k's squared safety cannot be below `safety2`
@@ -183,49 +188,44 @@ struct PolyhedronS {
     }
     return valid;
   }
-  // Quadrilaterals::DistanceToIn<kScalar, behind>, Quadrilaterals.h:302-350:
-  // the loop ends at the first quadrilateral whose hit leaves best finite.
-  // Written without early exits inside an iteration -- every lane evaluates
-  // every test of the quadrilateral with the same arithmetic and masks the
-  // outcome -- so that the lanes of a wave stay converged; `done` replays the
-  // reference's break.
+  // Quadrilaterals::DistanceToIn<kScalar, behind>, Quadrilaterals.h:302-350
+  // (early exits as the reference's scalar loop; a branch-free form that
+  // kept the waves converged measured 1.5x slower on the zoo workload).
   template <bool Behind>
   static VGA_HD double QDistanceToIn(const Quads &s, V3 p, V3 d) {
     double best = kInfinity;
-    bool done = false;
-    for (int i = 0; i < s.n && !done; ++i) {
-      const double *R = s.R(i);
-      const V3 nn{R[0], R[1], R[2]};
-      const double dist = Dot(p, nn) + R[3];
+    for (int i = 0; i < s.n; ++i) {
+      const V3 nn = s.N(i);
+      double distance = Dot(p, nn) + s.D(i);
+      if (!((Behind ? -distance : distance) >= 0)) continue;
       const double proj = Dot(d, nn);
-      const double t = dist / -proj;
-      const V3 x = p + d * t;
-      bool valid = ((Behind ? -dist : dist) >= 0) & ((Behind ? proj : -proj) >= 0);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) valid &= x.x * R[4 + 4 * j] + x.y * R[5 + 4 * j] + x.z * R[6 + 4 * j] + R[7 + 4 * j] >= 0;
-      best = valid ? t : best;
-      done = best < kInfinity;
+      if (!((Behind ? proj : -proj) >= 0)) continue;
+      distance /= -proj;
+      const V3 x = p + d * distance;
+      if (RayHits(s, i, x)) best = distance;
+      if (best < kInfinity) break;
     }
     return best;
   }
-  // Quadrilaterals::DistanceToOut<kScalar>, Quadrilaterals.h:435-490, branch
-  // free within an iteration as QDistanceToIn.
+  // Quadrilaterals::DistanceToOut<kScalar>, Quadrilaterals.h:435-490
   static VGA_HD double QDistanceToOut(const Quads &s, V3 p, V3 d, double zMin, double zMax) {
     double best = kInfinity;
-    const bool degenerate = zMin == zMax;
     for (int i = 0; i < s.n; ++i) {
       const V3 nn = s.N(i);
-      const double t0 = Dot(p, nn) + s.D(i);
+      double t = Dot(p, nn) + s.D(i);
+      if (!(t < 0)) continue;
       const double proj = Dot(d, nn);
-      const double t = t0 / -proj;
-      bool valid = (t0 < 0) & (proj >= 0) & (t < best);
-      if (degenerate) {
-        valid = valid && RayHits(s, i, p + t * d);
+      if (!(proj >= 0)) continue;
+      t /= -proj;
+      if (!(t < best)) continue;
+      bool valid;
+      if (zMin == zMax) {
+        valid = RayHits(s, i, p + t * d);
       } else {
         const double zp = p.z + t * d.z;
-        valid &= (zp >= zMin) & (zp < zMax);
+        valid = zp >= zMin && zp < zMax;
       }
-      best = valid ? t : best;
+      if (valid) best = t;
     }
     return best;
   }
@@ -325,13 +325,15 @@ struct PolyhedronS {
   }
 
   // DistanceToInZSegment, :387-428
-  static VGA_HD double DistanceToInSegment(const P &q, int k, V3 p, V3 d) {
+  // inner: false when RayNearSegment has ruled out inner hits (the inner
+  // shell would return kInfinity).
+  static VGA_HD double DistanceToInSegment(const P &q, int k, V3 p, V3 d, bool inner) {
     const Seg s = GetSeg(q, k);
     double distance = QDistanceToIn<false>(s.outer, p, d);
     if (distance < kInfinity) return distance;
     if (HasPhi(q)) distance = QDistanceToIn<false>(s.phi, p, d);
     if (distance < kInfinity) return distance;
-    if (s.hasInner) distance = QDistanceToIn<true>(s.inner, p, d);
+    if (s.hasInner) distance = inner ? QDistanceToIn<true>(s.inner, p, d) : kInfinity;
     return distance;
   }
   // DistanceToOutZSegment, :430-476
@@ -366,8 +368,13 @@ struct PolyhedronS {
     if (s.hasInner && s.inner.n > 0) si = QDistanceSquared(s.inner, phiIndex, p);
     return Min(si, so);
   }
-  // The endcap tests of ScalarDistanceToEndcaps / ScalarSafetyToEndcapsSquared.
-  static VGA_HD bool OnEndcap(const P &q, const Seg &s, V3 x) {
+  // The endcap tests of ScalarDistanceToEndcaps / ScalarSafetyToEndcapsSquared
+  // on end segment k.  A point more than 1e-6 mm outside the segment's corner
+  // radius is outside some outer plane, so the outer test is skipped there.
+  static VGA_HD bool OnEndcap(const P &q, int k, V3 x) {
+    const double rOut = SegRec(q, k)[8] + 1e-6;
+    if (x.x * x.x + x.y * x.y > rOut * rOut) return false;
+    const Seg s = GetSeg(q, k);
     if (!QContains(s.outer, x)) return false;
     if (s.hasInner && QContains(s.inner, x)) return false;
     if (HasPhi(q) && InPhiWedge(q, s, x)) return false;
@@ -381,7 +388,7 @@ struct PolyhedronS {
     const double zPlane = useFirst ? Z(q, 0) : Z(q, n);
     const double t = (zPlane - p.z) / d.z;
     if (t < 0 || t >= distance) return;
-    if (OnEndcap(q, GetSeg(q, useFirst ? 0 : n - 1), p + t * d)) distance = t;
+    if (OnEndcap(q, useFirst ? 0 : n - 1, p + t * d)) distance = t;
   }
   // ScalarSafetyToEndcapsSquared, :604-645
   static VGA_HD void SafetyToEndcapsSquared(const P &q, V3 p, double &distSq) {
@@ -392,7 +399,7 @@ struct PolyhedronS {
     const double t = isFirst ? firstD : lastD;
     const double tSq = t * t;
     if (tSq >= distSq) return;
-    if (OnEndcap(q, GetSeg(q, isFirst ? 0 : n - 1), V3{p.x, p.y, p.z + t})) distSq = tSq;
+    if (OnEndcap(q, isFirst ? 0 : n - 1, V3{p.x, p.y, p.z + t})) distSq = tSq;
   }
 
   // ScalarDistanceToInKernel, :832-913 (local point and direction)
@@ -406,12 +413,14 @@ struct PolyhedronS {
     double distance = kInfinity;
     if (d.z >= 0) {
       for (; k < n; ++k) {
-        distance = RayNearSegment(q, k, p, d) ? DistanceToInSegment(q, k, p, d) : kInfinity;
+        const int near = RayNearSegment(q, k, p, d);
+        distance = near ? DistanceToInSegment(q, k, p, d, near == 2) : kInfinity;
         if (distance >= 0 && distance < kInfinity) break;
       }
     } else {
       for (; k >= 0; --k) {
-        distance = RayNearSegment(q, k, p, d) ? DistanceToInSegment(q, k, p, d) : kInfinity;
+        const int near = RayNearSegment(q, k, p, d);
+        distance = near ? DistanceToInSegment(q, k, p, d, near == 2) : kInfinity;
         if (distance >= 0 && distance < kInfinity) break;
       }
     }

@@ -184,6 +184,61 @@ __device__ __forceinline__ void RunTask(const TaskDev t, const double *blk, uint
   }
 }
 
+// DistanceToIn of one tile with lane compaction (the Torus, whose exact
+// bounding-tube rejection discards most rays): every lane tests its items,
+// writes kInfinity for the rejected ones and appends the others to a
+// workgroup list in LDS; then the list is processed by the first lanes, so
+// the quartic runs on dense waves.  Called by every thread of the workgroup.
+template <class S, int IT>
+__device__ __forceinline__ void RunTileCompactDistIn(const TaskDev t, const double *blk, uint64_t tile_first) {
+  constexpr int kItems = kBatchBlock * IT;
+  __shared__ int count;
+  __shared__ int item[kItems];
+  __shared__ double qp[3][kItems], qd[3][kItems], qsm[kItems];
+  const auto &P = ParamView<S>::Get(blk);
+  if (threadIdx.x == 0) count = 0;
+  __syncthreads();
+  const uint64_t i0 = tile_first + (uint64_t)threadIdx.x * IT;
+  const bool full = i0 + IT <= t.n;
+  double px[IT], py[IT], pz[IT], dx[IT], dy[IT], dz[IT];
+  if (i0 < t.n) {
+    LoadAll<IT>(t, i0, full, t.x, px);
+    LoadAll<IT>(t, i0, full, t.y, py);
+    LoadAll<IT>(t, i0, full, t.z, pz);
+    LoadAll<IT>(t, i0, full, t.dx, dx);
+    LoadAll<IT>(t, i0, full, t.dy, dy);
+    LoadAll<IT>(t, i0, full, t.dz, dz);
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      const uint64_t i = i0 + k;
+      if (i >= t.n) break;
+      const V3 p = XfPointRt(t.xf, t.xfkind, V3{px[k], py[k], pz[k]});
+      const V3 d = XfDirRt(t.xf, t.xfkind, V3{dx[k], dy[k], dz[k]});
+      const double sm = t.sm ? t.sm[i] : kInfinity;
+      double r;
+      if (S::DistanceToInQuick(P, p, d, sm, r)) {
+        static_cast<double *>(t.out)[i] = r;
+      } else {
+        const int j = atomicAdd(&count, 1);
+        item[j] = (int)(i - tile_first);
+        qp[0][j] = p.x;
+        qp[1][j] = p.y;
+        qp[2][j] = p.z;
+        qd[0][j] = d.x;
+        qd[1][j] = d.y;
+        qd[2][j] = d.z;
+        qsm[j] = sm;
+      }
+    }
+  }
+  __syncthreads();
+  const int c = count;
+  for (int j = threadIdx.x; j < c; j += kBatchBlock) {
+    static_cast<double *>(t.out)[tile_first + item[j]] =
+        S::DistanceToIn(P, V3{qp[0][j], qp[1][j], qp[2][j]}, V3{qd[0][j], qd[1][j], qd[2][j]}, qsm[j]);
+  }
+}
+
 // Copies a task's parameter block to LDS (whole workgroup).
 __device__ __forceinline__ void Stage(double *stage, const double *src, int len) {
   for (int j = threadIdx.x; j < len; j += kBatchBlock) stage[j] = src[j];
@@ -228,6 +283,12 @@ __global__ __launch_bounds__(kBatchBlock) void BatchKernel(const TaskDev *__rest
       Stage(stage, params + t.param_off, t.param_len);
       __syncthreads();
       if (i0 < t.n) RunTask<IT, CLS>(t, stage, i0);
+    } else if constexpr (CLS == 2) {
+      if (t.query == kQDistIn) {  // uniform over the workgroup
+        RunTileCompactDistIn<TorusS, IT>(t, params + t.param_off, (tile - t.tile_begin) * kBatchBlock * IT);
+      } else if (i0 < t.n) {
+        RunTask<IT, CLS>(t, params + t.param_off, i0);
+      }
     } else {
       if (i0 < t.n) RunTask<IT, CLS>(t, params + t.param_off, i0);
     }

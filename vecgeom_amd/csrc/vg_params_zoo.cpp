@@ -219,6 +219,7 @@ std::vector<double> BuildPolyhedron(const double *c, int n, std::string *err) {
     // and outer corner radius.
     rec[7] = sc[ip].inner ? std::min(rmin[ip], rmin[ip + 1]) : 0.;
     rec[8] = std::max(std::max(rmax[ip], rmax[ip + 1]), std::max(rmin[ip], rmin[ip + 1])) / cosHalf;
+    rec[9] = std::max(rmin[ip], rmin[ip + 1]) / cosHalf;
     rec[4] = quadOff;
     for (int k = 0; k < sc[ip].nOuter; ++k) {
       double *q = blk.data() + quadOff + PolyhedronS::kQuadDoubles * k;
