@@ -99,6 +99,10 @@ struct PolyconeS {
   // The reference's polycone Inside reports inside or outside only.
   static VGA_HD int Inside(const P &q, V3 p) { return Contains(q, p) ? kInside : kOutside; }
 
+  // PolyconeImplementation.h:148-192.  (Run as a refilled walk, as the
+  // Polyhedron's DistanceToIn is, or with each lane skipping ahead over the
+  // sections it cannot hit, it measured slower: the cone code per section is
+  // short next to the walk's bookkeeping.)
   static VGA_HD double DistanceToIn(const P &q, V3 p, V3 v, double stepMax) {
     const int n = NSec(q);
     double distance = kInfinity;

@@ -188,26 +188,43 @@ struct PolyhedronS {
     }
     return valid;
   }
-  // Quadrilaterals::DistanceToIn<kScalar, behind>, Quadrilaterals.h:302-350
-  // (early exits as the reference's scalar loop; a branch-free form that
-  // kept the waves converged measured 1.5x slower on the zoo workload).
+  // Quadrilaterals::DistanceToIn<kScalar, behind>, Quadrilaterals.h:302-350.
+  // The same tests in the same order as the reference's scalar loop, run in
+  // two passes over chunks of 32 quadrilaterals: the first evaluates the
+  // cheap plane tests of every quadrilateral (a wave stays converged there)
+  // and records the survivors in a bit mask; the second visits only each
+  // lane's own survivors in index order for the division and the side-plane
+  // tests.  A wave then pays for the largest survivor count among its lanes
+  // instead of for every quadrilateral that any lane survives.
   template <bool Behind>
   static VGA_HD double QDistanceToIn(const Quads &s, V3 p, V3 d) {
     double best = kInfinity;
-    for (int i = 0; i < s.n; ++i) {
-      const V3 nn = s.N(i);
-      double distance = Dot(p, nn) + s.D(i);
-      if (!((Behind ? -distance : distance) >= 0)) continue;
-      const double proj = Dot(d, nn);
-      if (!((Behind ? proj : -proj) >= 0)) continue;
-      distance /= -proj;
-      const V3 x = p + d * distance;
-      if (RayHits(s, i, x)) best = distance;
-      if (best < kInfinity) break;
+    for (int base = 0; base < s.n; base += 32) {
+      const int m = s.n - base < 32 ? s.n - base : 32;
+      uint32_t mask = 0;
+      for (int k = 0; k < m; ++k) {
+        const V3 nn = s.N(base + k);
+        const double distance = Dot(p, nn) + s.D(base + k);
+        const double proj = Dot(d, nn);
+        const bool ok = (Behind ? -distance : distance) >= 0 && (Behind ? proj : -proj) >= 0;
+        mask |= (uint32_t)ok << k;
+      }
+      while (mask) {
+        const int i = base + __builtin_ctz(mask);
+        mask &= mask - 1;
+        const V3 nn = s.N(i);
+        double distance = Dot(p, nn) + s.D(i);
+        distance /= -Dot(d, nn);
+        const V3 x = p + d * distance;
+        if (RayHits(s, i, x)) best = distance;
+        if (best < kInfinity) return best;
+      }
     }
     return best;
   }
-  // Quadrilaterals::DistanceToOut<kScalar>, Quadrilaterals.h:435-490
+  // Quadrilaterals::DistanceToOut<kScalar>, Quadrilaterals.h:435-490 (a
+  // two-pass form as in QDistanceToIn measured slower here: from inside, about
+  // half of the quadrilaterals are candidates for every lane).
   static VGA_HD double QDistanceToOut(const Quads &s, V3 p, V3 d, double zMin, double zMax) {
     double best = kInfinity;
     for (int i = 0; i < s.n; ++i) {
@@ -402,30 +419,78 @@ struct PolyhedronS {
     if (OnEndcap(q, isFirst ? 0 : n - 1, V3{p.x, p.y, p.z + t})) distSq = tSq;
   }
 
-  // ScalarDistanceToInKernel, :832-913 (local point and direction)
-  static VGA_HD double DistanceToIn(const P &q, V3 p, V3 d, double stepMax) {
+  // ScalarDistanceToInKernel, :832-913 (local point and direction), split
+  // into a walk: DistInBegin (bounding tube, start segment, containment),
+  // DistInStep (one z segment's quadrilaterals), DistInEnd (endcaps, step
+  // limit).  The reference walks the segments in the direction of d.z; a
+  // segment the ray cannot come near yields kInfinity there, and the walk
+  // here skips ahead over such segments.  DistanceToIn runs the walk on one
+  // lane; the refill kernel interleaves the walks of a wave's lanes so that
+  // every DistInStep runs on as many lanes as possible.
+  static constexpr bool kHasWalk = true;
+  struct DistInWalk {
+    V3 p, d;
+    double stepMax, distance;
+    int k, step, near;
+    bool final;
+  };
+  // Moves w.k to the next segment (from w.k on) that the ray can hit; false
+  // when there is none.
+  static VGA_HD bool DistInAdvance(const P &q, DistInWalk &w) {
+    const int n = NSeg(q);
+    int near = RayNearSegment(q, w.k, w.p, w.d);
+    while (!near) {
+      w.k += w.step;
+      if (w.k < 0 || w.k >= n) return false;
+      near = RayNearSegment(q, w.k, w.p, w.d);
+    }
+    w.near = near;
+    return true;
+  }
+  // True when the query is resolved without a segment step.
+  static VGA_HD bool DistInBegin(const P &q, V3 p, V3 d, double stepMax, DistInWalk &w) {
+    w.p = p;
+    w.d = d;
+    w.stepMax = stepMax;
+    w.final = true;
     const V3 bp = BoundsPoint(q, p);
     const bool inBounds = TubeS::Contains(Tube(q), bp);
-    if (!inBounds && TubeS::DistanceToIn(Tube(q), bp, d, stepMax) == kInfinity) return kInfinity;
-    const int n = NSeg(q);
-    int k = ClampZ(q, FindZ(q, p.z));
-    if (inBounds && SegmentContains(q, p, k)) return -1.;
-    double distance = kInfinity;
-    if (d.z >= 0) {
-      for (; k < n; ++k) {
-        const int near = RayNearSegment(q, k, p, d);
-        distance = near ? DistanceToInSegment(q, k, p, d, near == 2) : kInfinity;
-        if (distance >= 0 && distance < kInfinity) break;
-      }
-    } else {
-      for (; k >= 0; --k) {
-        const int near = RayNearSegment(q, k, p, d);
-        distance = near ? DistanceToInSegment(q, k, p, d, near == 2) : kInfinity;
-        if (distance >= 0 && distance < kInfinity) break;
-      }
+    if (!inBounds && TubeS::DistanceToIn(Tube(q), bp, d, stepMax) == kInfinity) {
+      w.distance = kInfinity;
+      return true;
     }
-    DistanceToEndcaps<false>(q, p, d, distance);
-    return distance < stepMax ? distance : stepMax;
+    w.k = ClampZ(q, FindZ(q, p.z));
+    if (inBounds && SegmentContains(q, p, w.k)) {
+      w.distance = -1.;
+      return true;
+    }
+    w.final = false;
+    w.step = d.z >= 0 ? 1 : -1;
+    w.distance = kInfinity;
+    return !DistInAdvance(q, w);
+  }
+  // True when the walk is over.
+  static VGA_HD bool DistInStep(const P &q, DistInWalk &w) {
+    w.distance = DistanceToInSegment(q, w.k, w.p, w.d, w.near == 2);
+    if (w.distance >= 0 && w.distance < kInfinity) return true;
+    w.k += w.step;
+    if (w.k < 0 || w.k >= NSeg(q)) return true;
+    if (DistInAdvance(q, w)) return false;
+    w.distance = kInfinity;
+    return true;
+  }
+  static VGA_HD double DistInEnd(const P &q, DistInWalk &w) {
+    if (w.final) return w.distance;
+    double distance = w.distance;
+    DistanceToEndcaps<false>(q, w.p, w.d, distance);
+    return distance < w.stepMax ? distance : w.stepMax;
+  }
+  static VGA_HD double DistanceToIn(const P &q, V3 p, V3 d, double stepMax) {
+    DistInWalk w;
+    if (!DistInBegin(q, p, d, stepMax, w))
+      while (!DistInStep(q, w)) {
+      }
+    return DistInEnd(q, w);
   }
 
   // ScalarDistanceToOutKernel, :946-1004
@@ -456,6 +521,8 @@ struct PolyhedronS {
     double safety = kInfinity;
     const int k = ClampZ(q, FindZ(q, p.z));
     const int phiIndex = FindPhi(q, p);
+    // (Skipping ahead over pruned segments per lane, as DistanceToIn does,
+    // measured slower here: the segment computation is short.)
     for (int z = k; z < n;) {
       if (!SegmentSafetyAtLeast(q, z, p, safety)) safety = Min(safety, SafetySegmentSquared(q, z, phiIndex, p));
       ++z;

@@ -36,6 +36,7 @@
 #include <vector>
 
 #include "shapes.hpp"
+#include "vg_walk.hpp"
 #include "vg_internal.hpp"
 
 namespace vga {
@@ -157,6 +158,15 @@ constexpr int ClassOf(int type) {
 // takes it as given, the compute-heavy classes at most 2.
 constexpr int ClassIT(int c, int it) { return c == 0 ? it : (it < 2 ? it : 2); }
 
+// DistanceToIn tasks of the Polyhedron in the staged class run with lane
+// refill (vg_walk.hpp) over tiles kWalkTileMul times longer, so that each
+// wave refills its lanes from 4 * 64 * IT queries.
+constexpr int kWalkTileMul = 4;
+constexpr bool WalkTask(int type, int query) { return query == kQDistIn && type == VG_POLYHEDRON; }
+constexpr uint64_t TileItems(int c, int it, int type, int query) {
+  return (uint64_t)kBatchBlock * ClassIT(c, it) * ((c == kStagedClass && WalkTask(type, query)) ? kWalkTileMul : 1);
+}
+
 template <int IT, int CLS>
 __device__ __forceinline__ void RunTask(const TaskDev t, const double *blk, uint64_t i0) {
   if constexpr (CLS == 0) {
@@ -239,6 +249,32 @@ __device__ __forceinline__ void RunTileCompactDistIn(const TaskDev t, const doub
   }
 }
 
+__device__ __forceinline__ double LoadOne(const TaskDev &t, const void *a, uint64_t i) {
+  return t.in_type == VG_F32 ? (double)static_cast<const float *>(a)[i] : static_cast<const double *>(a)[i];
+}
+
+// DistanceToIn of items [first, first + count) of a walk task with lane
+// refill: each of the workgroup's waves takes a quarter of the range.
+// Called by every thread of the workgroup.
+template <class S>
+__device__ __forceinline__ void RunTileWalk(const TaskDev &t, const double *blk, uint64_t first, uint64_t count) {
+  constexpr int kWaves = kBatchBlock / 64;
+  const uint64_t per = (count + kWaves - 1) / kWaves;
+  const uint64_t lim = first + count < t.n ? first + count : t.n;
+  const uint64_t b = first + (uint64_t)(threadIdx.x / 64) * per;
+  const uint64_t begin = b < lim ? b : lim;
+  const uint64_t end = begin + per < lim ? begin + per : lim;
+  double *out = static_cast<double *>(t.out);
+  WalkRefill<S>(
+      ParamView<S>::Get(blk), begin, end,
+      [&](uint64_t i, V3 &p, V3 &d, double &sm) {
+        p = XfPointRt(t.xf, t.xfkind, V3{LoadOne(t, t.x, i), LoadOne(t, t.y, i), LoadOne(t, t.z, i)});
+        d = XfDirRt(t.xf, t.xfkind, V3{LoadOne(t, t.dx, i), LoadOne(t, t.dy, i), LoadOne(t, t.dz, i)});
+        sm = t.sm ? t.sm[i] : kInfinity;
+      },
+      [&](uint64_t i, double r) { out[i] = r; });
+}
+
 // Copies a task's parameter block to LDS (whole workgroup).
 __device__ __forceinline__ void Stage(double *stage, const double *src, int len) {
   for (int j = threadIdx.x; j < len; j += kBatchBlock) stage[j] = src[j];
@@ -282,7 +318,12 @@ __global__ __launch_bounds__(kBatchBlock) void BatchKernel(const TaskDev *__rest
       extern __shared__ double stage[];
       Stage(stage, params + t.param_off, t.param_len);
       __syncthreads();
-      if (i0 < t.n) RunTask<IT, CLS>(t, stage, i0);
+      if (WalkTask(t.type, t.query)) {  // uniform over the workgroup
+        const uint64_t per = TileItems(CLS, IT, t.type, t.query);
+        RunTileWalk<PolyhedronS>(t, stage, (tile - t.tile_begin) * per, per);
+      } else if (i0 < t.n) {
+        RunTask<IT, CLS>(t, stage, i0);
+      }
     } else if constexpr (CLS == 2) {
       if (t.query == kQDistIn) {  // uniform over the workgroup
         RunTileCompactDistIn<TorusS, IT>(t, params + t.param_off, (tile - t.tile_begin) * kBatchBlock * IT);
@@ -365,9 +406,9 @@ int TaskClass(const TaskDev &t) {
 void Schedule(vg_batch *b) {
   for (int c = 0; c < kClasses; ++c) {
     uint64_t tile = 0;
-    const uint64_t per = (uint64_t)kBatchBlock * ClassIT(c, b->it);
     for (int k = b->first[c]; k < b->first[c + 1]; ++k) {
       TaskDev &x = b->host[k];
+      const uint64_t per = TileItems(c, b->it, x.type, x.query);
       x.tile_begin = tile;
       tile += (x.n + per - 1) / per;
     }
@@ -400,6 +441,7 @@ int Launch(vg_batch *b, hipStream_t s) {
   if (classes > 1) rc = CheckHip(hipEventRecord(b->fork, s), "batch fork");
   int nside = 0;
   bool first = true;
+  // (Launching the heaviest classes first measured slower on the zoo step.)
   for (int c = 0; c < kClasses && !rc; ++c) {
     const uint64_t nt = b->ntiles[c];
     if (nt == 0) continue;

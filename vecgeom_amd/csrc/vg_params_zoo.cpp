@@ -185,10 +185,20 @@ std::vector<double> BuildPolyhedron(const double *c, int n, std::string *err) {
   const int zOff = PolyhedronS::kTubeOff + ((tubeDoubles + 1) & ~1);
   const int phiOff = zOff + ((nz + 1) & ~1);
   const int segOff = phiOff + ((3 * (sides + 1) + 1) & ~1);
-  int quadOff = segOff + PolyhedronS::kSegDoubles * nseg;
-  int nquads = 0;
-  for (auto &s : sc) nquads += s.nOuter + s.nPhi + s.nInner;
-  std::vector<double> blk(quadOff + (size_t)PolyhedronS::kQuadDoubles * nquads, 0.);
+  // Segment k's quadrilaterals start at an offset of k * sides (mod 32)
+  // doubles: with 33-double records, the lanes of a wave that read the same
+  // quadrilateral slot of different segments (or different quadrilaterals of
+  // up to 32 / sides segments) then read different LDS banks.
+  const int quadBase = segOff + PolyhedronS::kSegDoubles * nseg;
+  std::vector<int> segQuadOff(nseg);
+  int end = quadBase;
+  for (int ip = 0; ip < nseg; ++ip) {
+    const int want = (ip * sides) & 31;
+    end += (want - end) & 31;
+    segQuadOff[ip] = end;
+    end += PolyhedronS::kQuadDoubles * (sc[ip].nOuter + sc[ip].nPhi + sc[ip].nInner);
+  }
+  std::vector<double> blk(end, 0.);
   blk[0] = nseg; blk[1] = sides; blk[2] = hasPhi; blk[3] = largePhi; blk[4] = z[0] + 0.5 * tubeZ;
   blk[5] = zOff; blk[6] = phiOff; blk[7] = segOff; blk[8] = (double)blk.size();
   std::memcpy(blk.data() + PolyhedronS::kTubeOff, &tube, sizeof(TubeP));
@@ -220,6 +230,7 @@ std::vector<double> BuildPolyhedron(const double *c, int n, std::string *err) {
     rec[7] = sc[ip].inner ? std::min(rmin[ip], rmin[ip + 1]) : 0.;
     rec[8] = std::max(std::max(rmax[ip], rmax[ip + 1]), std::max(rmin[ip], rmin[ip + 1])) / cosHalf;
     rec[9] = std::max(rmin[ip], rmin[ip + 1]) / cosHalf;
+    int quadOff = segQuadOff[ip];
     rec[4] = quadOff;
     for (int k = 0; k < sc[ip].nOuter; ++k) {
       double *q = blk.data() + quadOff + PolyhedronS::kQuadDoubles * k;

@@ -18,6 +18,7 @@
 #include <cstring>
 
 #include "shapes.hpp"
+#include "vg_walk.hpp"
 #include "vg_internal.hpp"
 
 namespace vga {
@@ -74,17 +75,9 @@ constexpr bool NeedsStep(int q) { return q == kQDistIn || q == kQDistOut || q ==
 // and every lane reads it from there.
 constexpr int kMaxStageDoubles = 6144;  // 48 KiB of LDS
 
-template <class S, int XK, int Q, int IT, bool Stage>
-__global__ __launch_bounds__(kBlock) void ShapeQueryKernel(const typename S::P Pg, const Xf X, const QArgs a,
-                                                           int stage_len) {
-  typename S::P P = Pg;
-  if constexpr (Stage) {
-    extern __shared__ double stage[];
-    for (int k = threadIdx.x; k < stage_len; k += kBlock) stage[k] = Pg.blk[k];
-    __syncthreads();
-    P = typename S::P{stage};
-  }
-  const uint64_t i0 = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * IT;
+// One lane's IT consecutive queries starting at i0.
+template <class S, int XK, int Q, int IT>
+__device__ __forceinline__ void QueryItems(const typename S::P &P, const Xf &X, const QArgs &a, uint64_t i0) {
   if (i0 >= a.n) return;
   double px[IT], py[IT], pz[IT], dx[IT], dy[IT], dz[IT], sm[IT];
   const bool full = i0 + IT <= a.n;
@@ -191,6 +184,23 @@ __global__ __launch_bounds__(kBlock) void ShapeQueryKernel(const typename S::P P
   }
 }
 
+// One workgroup per kBlock * IT queries.  (A grid of only the resident
+// workgroups, each staging once and striding over the tiles, measured slower
+// for the expensive queries: the per-tile cost varies too much for a static
+// split, and the hardware's dispatch of short workgroups balances it.)
+template <class S, int XK, int Q, int IT, bool Stage>
+__global__ __launch_bounds__(kBlock) void ShapeQueryKernel(const typename S::P Pg, const Xf X, const QArgs a,
+                                                           int stage_len) {
+  if constexpr (Stage) {
+    extern __shared__ double stage[];
+    for (int k = threadIdx.x; k < stage_len; k += kBlock) stage[k] = Pg.blk[k];
+    __syncthreads();
+    QueryItems<S, XK, Q, IT>(typename S::P{stage}, X, a, ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * IT);
+  } else {
+    QueryItems<S, XK, Q, IT>(Pg, X, a, ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * IT);
+  }
+}
+
 template <class S, class = void>
 struct HasQuick {
   static constexpr bool value = false;
@@ -249,6 +259,44 @@ __global__ __launch_bounds__(kBlock) void CompactDistInKernel(const typename S::
   }
 }
 
+template <class S, class = void>
+struct HasWalk {
+  static constexpr bool value = false;
+};
+template <class S>
+struct HasWalk<S, decltype((void)S::kHasWalk)> {
+  static constexpr bool value = S::kHasWalk;
+};
+
+// DistanceToIn with lane refill (vg_walk.hpp) for shapes whose DistanceToIn
+// is a walk of variable length: each wave owns kWalkPerLane * 64 consecutive
+// queries.
+constexpr int kWalkPerLane = 8;
+
+template <class S, int XK, bool Stage>
+__global__ __launch_bounds__(kBlock) void WalkDistInKernel(const typename S::P Pg, const Xf X, const QArgs a,
+                                                           int stage_len) {
+  typename S::P P = Pg;
+  if constexpr (Stage) {
+    extern __shared__ double stage[];
+    for (int k = threadIdx.x; k < stage_len; k += kBlock) stage[k] = Pg.blk[k];
+    __syncthreads();
+    P = typename S::P{stage};
+  }
+  constexpr uint64_t kChunk = 64 * kWalkPerLane;
+  const uint64_t begin = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / 64 * kChunk;
+  const uint64_t end = begin + kChunk < a.n ? begin + kChunk : a.n;
+  double *out = reinterpret_cast<double *>(a.out);
+  WalkRefill<S>(
+      P, begin, end,
+      [&](uint64_t i, V3 &p, V3 &d, double &sm) {
+        p = XfPoint<XK>(X, V3{a.x[i], a.y[i], a.z[i]});
+        d = XfDir<XK>(X, V3{a.dx[i], a.dy[i], a.dz[i]});
+        sm = a.sm ? a.sm[i] : kInfinity;
+      },
+      [&](uint64_t i, double r) { out[i] = r; });
+}
+
 // Plain FP64 form (the benchmark denominator): one thread per query, the
 // placement record and shape parameters read from global memory through the
 // placed-volume id (as the reference's device virtual call chases its object
@@ -292,6 +340,19 @@ int LaunchQ(const typename S::P &P, const Xf &X, const QArgs &a, int stage_len, 
   bool al = Aligned16(a.x) && Aligned16(a.y) && Aligned16(a.z) && Aligned16(a.out);
   if (NeedsDir(Q)) al = al && Aligned16(a.dx) && Aligned16(a.dy) && Aligned16(a.dz);
   if (NeedsStep(Q) && a.sm) al = al && Aligned16(a.sm);
+  if constexpr (Q == kQDistIn && HasWalk<S>::value) {
+    const uint64_t waves = (a.n + 64 * kWalkPerLane - 1) / (64 * kWalkPerLane);
+    const uint64_t blocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
+    if constexpr (ParamView<S>::kIndirect) {
+      if (stage_len > 0 && stage_len <= kMaxStageDoubles) {
+        hipLaunchKernelGGL((WalkDistInKernel<S, XK, true>), dim3((unsigned)blocks), dim3(kBlock),
+                           stage_len * sizeof(double), s, P, X, a, stage_len);
+        return CheckHip(hipGetLastError(), "shape query launch");
+      }
+    }
+    hipLaunchKernelGGL((WalkDistInKernel<S, XK, false>), dim3((unsigned)blocks), dim3(kBlock), 0, s, P, X, a, 0);
+    return CheckHip(hipGetLastError(), "shape query launch");
+  }
   if constexpr (Q == kQDistIn && HasQuick<S>::value) {
     const uint64_t blocks = (a.n + kBlock - 1) / kBlock;
     if constexpr (ParamView<S>::kIndirect) {
