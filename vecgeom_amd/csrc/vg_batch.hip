@@ -36,6 +36,7 @@
 #include <vector>
 
 #include "shapes.hpp"
+#include "vg_stage.hpp"
 #include "vg_walk.hpp"
 #include "vg_internal.hpp"
 
@@ -277,7 +278,7 @@ __device__ __forceinline__ void RunTileWalk(const TaskDev &t, const double *blk,
 
 // Copies a task's parameter block to LDS (whole workgroup).
 __device__ __forceinline__ void Stage(double *stage, const double *src, int len) {
-  for (int j = threadIdx.x; j < len; j += kBatchBlock) stage[j] = src[j];
+  StageBlock(stage, src, len, kBatchBlock);
 }
 
 // Tile -> task: the last task whose first tile is <= tile (tasks sorted by
@@ -315,7 +316,7 @@ __global__ __launch_bounds__(kBatchBlock) void BatchKernel(const TaskDev *__rest
     const TaskDev t = tasks[k];
     const uint64_t i0 = ((tile - t.tile_begin) * kBatchBlock + threadIdx.x) * IT;
     if constexpr (CLS == kStagedClass) {
-      extern __shared__ double stage[];
+      extern __shared__ __align__(16) double stage[];
       Stage(stage, params + t.param_off, t.param_len);
       __syncthreads();
       if (WalkTask(t.type, t.query)) {  // uniform over the workgroup
@@ -342,7 +343,7 @@ template <int IT, int CLS>
 __global__ __launch_bounds__(kBatchBlock) void TaskKernel(const TaskDev t, const double *__restrict__ params) {
   const uint64_t i0 = ((uint64_t)blockIdx.x * kBatchBlock + threadIdx.x) * IT;
   if constexpr (CLS == kStagedClass) {
-    extern __shared__ double stage[];
+    extern __shared__ __align__(16) double stage[];
     if (t.param_len <= kMaxStageDoubles) {
       Stage(stage, params + t.param_off, t.param_len);
       __syncthreads();

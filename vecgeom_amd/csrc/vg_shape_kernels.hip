@@ -18,6 +18,7 @@
 #include <cstring>
 
 #include "shapes.hpp"
+#include "vg_stage.hpp"
 #include "vg_walk.hpp"
 #include "vg_internal.hpp"
 
@@ -192,8 +193,8 @@ template <class S, int XK, int Q, int IT, bool Stage>
 __global__ __launch_bounds__(kBlock) void ShapeQueryKernel(const typename S::P Pg, const Xf X, const QArgs a,
                                                            int stage_len) {
   if constexpr (Stage) {
-    extern __shared__ double stage[];
-    for (int k = threadIdx.x; k < stage_len; k += kBlock) stage[k] = Pg.blk[k];
+    extern __shared__ __align__(16) double stage[];
+    StageBlock(stage, Pg.blk, stage_len, kBlock);
     __syncthreads();
     QueryItems<S, XK, Q, IT>(typename S::P{stage}, X, a, ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * IT);
   } else {
@@ -221,8 +222,8 @@ __global__ __launch_bounds__(kBlock) void CompactDistInKernel(const typename S::
                                                               int stage_len) {
   typename S::P P = Pg;
   if constexpr (Stage) {
-    extern __shared__ double stage[];
-    for (int k = threadIdx.x; k < stage_len; k += kBlock) stage[k] = Pg.blk[k];
+    extern __shared__ __align__(16) double stage[];
+    StageBlock(stage, Pg.blk, stage_len, kBlock);
     P = typename S::P{stage};
   }
   __shared__ int count;
@@ -278,8 +279,8 @@ __global__ __launch_bounds__(kBlock) void WalkDistInKernel(const typename S::P P
                                                            int stage_len) {
   typename S::P P = Pg;
   if constexpr (Stage) {
-    extern __shared__ double stage[];
-    for (int k = threadIdx.x; k < stage_len; k += kBlock) stage[k] = Pg.blk[k];
+    extern __shared__ __align__(16) double stage[];
+    StageBlock(stage, Pg.blk, stage_len, kBlock);
     __syncthreads();
     P = typename S::P{stage};
   }
