@@ -52,6 +52,17 @@ def profiled_traffic(shape, query, n):
     return None
 
 
+# DistanceToIn kernels other than the per-lane ShapeQueryKernel.
+DIST_IN_KERNEL = {"Polyhedron": "WalkDistInKernel", "Torus": "CompactDistInKernel"}
+
+
+def kernel_name(shape, query):
+    """The device kernel that serves one (shape, query) launch of the step."""
+    if query == "dist_in" and shape in DIST_IN_KERNEL:
+        return f"{DIST_IN_KERNEL[shape]}<{shape}>"
+    return f"ShapeQueryKernel<{shape},{query}>"
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -513,7 +524,7 @@ def main():
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom["shape"], dom["query"], dom["n"]),
-            "kernel": f"ShapeQueryKernel<{dom['shape']},{dom['query']}>", "avg_launch_us": round(1e3 * dom_ms, 3),
+            "kernel": kernel_name(dom['shape'], dom['query']), "avg_launch_us": round(1e3 * dom_ms, 3),
             "bytes_per_query": BYTES[dom["query"]], "queries_per_launch": dom["n"], "bytes_per_launch": dom_bytes}
 
     log(f"timed shape steps done: {value:.3f} Gqueries/s")

@@ -148,6 +148,12 @@ constexpr int kClasses = 5;
 constexpr int kStagedClass = 3;
 constexpr int kMaxStageDoubles = 6144;  // 48 KiB of LDS
 
+// Occupancy floor (waves per SIMD) of the staged class's kernel: latency
+// bound, it runs best at 6 waves (a few spilled registers) rather than at
+// the 4 its 121 VGPRs allow; measured 88 -> 100 G queries/s on the batched
+// zoo step (7 waves: 99, 8: 93).  The Torus class measured best unbounded.
+constexpr int ClassWaves(int c) { return c == kStagedClass ? 6 : 1; }
+
 constexpr int ClassOf(int type) {
   return (type == VG_CONE || type == VG_SPHERE)           ? 1
          : (type == VG_TORUS)                              ? 2
@@ -300,7 +306,7 @@ __device__ __forceinline__ int FindTask(const TaskDev *tasks, int ntasks, uint64
 // persistent loop around their bodies keeps the task's uniform values live
 // across iterations and more than doubles their VGPR count.
 template <int IT, int CLS>
-__global__ __launch_bounds__(kBatchBlock) void BatchKernel(const TaskDev *__restrict__ tasks, int ntasks,
+__global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(ClassWaves(CLS)))) void BatchKernel(const TaskDev *__restrict__ tasks, int ntasks,
                                                            uint64_t ntiles, const double *__restrict__ params) {
   if constexpr (CLS == 0) {
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -340,7 +346,7 @@ __global__ __launch_bounds__(kBatchBlock) void BatchKernel(const TaskDev *__rest
 // One task passed by value (kernel argument): the single-query form of the
 // same code, for callers that do not keep a batch.
 template <int IT, int CLS>
-__global__ __launch_bounds__(kBatchBlock) void TaskKernel(const TaskDev t, const double *__restrict__ params) {
+__global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(ClassWaves(CLS)))) void TaskKernel(const TaskDev t, const double *__restrict__ params) {
   const uint64_t i0 = ((uint64_t)blockIdx.x * kBatchBlock + threadIdx.x) * IT;
   if constexpr (CLS == kStagedClass) {
     extern __shared__ __align__(16) double stage[];

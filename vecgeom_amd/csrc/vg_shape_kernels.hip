@@ -76,6 +76,16 @@ constexpr bool NeedsStep(int q) { return q == kQDistIn || q == kQDistOut || q ==
 // and every lane reads it from there.
 constexpr int kMaxStageDoubles = 6144;  // 48 KiB of LDS
 
+// Occupancy floors (waves per SIMD) for the staged kernels, which are bound
+// by latency (LDS reads, dependent FP64 chains, divergence) rather than by
+// issue: left alone the compiler spends up to 128 VGPRs on them and holds a
+// SIMD to 4 waves.  Asking for 6 (7 for the refill walk) costs a few
+// spilled registers in the safeties and measured 4-13 % faster per launch on
+// the zoo workload (Polyhedron DistanceToIn 249 -> 216 us, SafetyToIn 214 ->
+// 200 us); 8 measured slower.
+constexpr int kStagedWaves = 6;
+constexpr int kWalkWaves = 7;
+
 // One lane's IT consecutive queries starting at i0.
 template <class S, int XK, int Q, int IT>
 __device__ __forceinline__ void QueryItems(const typename S::P &P, const Xf &X, const QArgs &a, uint64_t i0) {
@@ -190,7 +200,8 @@ __device__ __forceinline__ void QueryItems(const typename S::P &P, const Xf &X, 
 // for the expensive queries: the per-tile cost varies too much for a static
 // split, and the hardware's dispatch of short workgroups balances it.)
 template <class S, int XK, int Q, int IT, bool Stage>
-__global__ __launch_bounds__(kBlock) void ShapeQueryKernel(const typename S::P Pg, const Xf X, const QArgs a,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ParamView<S>::kIndirect ? kStagedWaves : 1)))
+void ShapeQueryKernel(const typename S::P Pg, const Xf X, const QArgs a,
                                                            int stage_len) {
   if constexpr (Stage) {
     extern __shared__ __align__(16) double stage[];
@@ -275,7 +286,7 @@ struct HasWalk<S, decltype((void)S::kHasWalk)> {
 constexpr int kWalkPerLane = 8;
 
 template <class S, int XK, bool Stage>
-__global__ __launch_bounds__(kBlock) void WalkDistInKernel(const typename S::P Pg, const Xf X, const QArgs a,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWalkWaves))) void WalkDistInKernel(const typename S::P Pg, const Xf X, const QArgs a,
                                                            int stage_len) {
   typename S::P P = Pg;
   if constexpr (Stage) {
