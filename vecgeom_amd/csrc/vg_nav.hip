@@ -147,8 +147,16 @@ struct TraceArgs {
   int32_t max_steps;
 };
 
+// Occupancy floor of the fused trace kernel over the basic shape set: it is
+// latency-bound, and 5 waves per SIMD (instead of the 3 its 145 VGPRs allow,
+// at the price of spilled registers) measured 0.91 -> 1.01 G steps/s on the
+// nested geometry (4 waves: 1.00, 6: 0.89).  The per-step and serving
+// kernels measured no better with a floor.
+constexpr int TraceWaves(uint32_t m) { return m == kBasicShapes ? 5 : 1; }
+
 template <int D, uint32_t M>
-__global__ __launch_bounds__(kNavBlock) void TraceKernel(const NavDev g, const TraceArgs a) {
+__global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(TraceWaves(M)))) void TraceKernel(
+    const NavDev g, const TraceArgs a) {
   const uint64_t i = (uint64_t)blockIdx.x * kNavBlock + threadIdx.x;
   if (i >= a.n) return;
   V3 p{a.x[i], a.y[i], a.z[i]};
