@@ -52,15 +52,15 @@ def profiled_traffic(shape, query, n):
     return None
 
 
-# DistanceToIn kernels other than the per-lane ShapeQueryKernel.
-DIST_IN_KERNEL = {"Polyhedron": "WalkDistInKernel", "Torus": "CompactDistInKernel"}
+# Kernels other than the per-lane ShapeQueryKernel.
+KERNEL = {("Polyhedron", "dist_in"): "WalkKernel<Polyhedron,DistInOps>",
+          ("Polyhedron", "dist_out"): "WalkKernel<Polyhedron,DistOutOps>",
+          ("Torus", "dist_in"): "CompactDistInKernel<Torus>"}
 
 
 def kernel_name(shape, query):
     """The device kernel that serves one (shape, query) launch of the step."""
-    if query == "dist_in" and shape in DIST_IN_KERNEL:
-        return f"{DIST_IN_KERNEL[shape]}<{shape}>"
-    return f"ShapeQueryKernel<{shape},{query}>"
+    return KERNEL.get((shape, query), f"ShapeQueryKernel<{shape},{query}>")
 
 
 def log(msg):

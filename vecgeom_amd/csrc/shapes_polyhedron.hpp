@@ -492,28 +492,56 @@ struct PolyhedronS {
       }
     return DistInEnd(q, w);
   }
+  struct DistInOps {
+    using State = DistInWalk;
+    static VGA_HD bool Begin(const P &q, V3 p, V3 d, double sm, State &w) { return DistInBegin(q, p, d, sm, w); }
+    static VGA_HD bool Step(const P &q, State &w) { return DistInStep(q, w); }
+    static VGA_HD double End(const P &q, State &w) { return DistInEnd(q, w); }
+  };
 
-  // ScalarDistanceToOutKernel, :946-1004
-  static VGA_HD double DistanceToOut(const P &q, V3 p, V3 d, double stepMax) {
-    const int n = NSeg(q);
-    int k = ClampZ(q, FindZ(q, p.z));
-    double distance = kInfinity;
-    if (d.z >= 0) {
-      for (; k < n; ++k) {
-        distance = DistanceToOutSegment(q, k, p, d);
-        if (distance >= 0 && distance < kInfinity) break;
-        if (Z(q, k) - p.z > distance) break;
-      }
-    } else {
-      for (; k >= 0; --k) {
-        distance = DistanceToOutSegment(q, k, p, d);
-        if (distance >= 0 && distance < kInfinity) break;
-        if (p.z - Z(q, k) > distance) break;
-      }
-    }
-    DistanceToEndcaps<true>(q, p, d, distance);
-    return distance < stepMax ? distance : stepMax;
+  // ScalarDistanceToOutKernel, :946-1004 (local point and direction), as a
+  // walk over the z segments in the direction of d.z like DistanceToIn:
+  // DistOutBegin (start segment), DistOutStep (one segment and the
+  // reference's two stopping tests), DistOutEnd (endcaps, step limit).
+  struct DistOutWalk {
+    V3 p, d;
+    double stepMax, distance;
+    int k, step;
+  };
+  static VGA_HD bool DistOutBegin(const P &q, V3 p, V3 d, double stepMax, DistOutWalk &w) {
+    w.p = p;
+    w.d = d;
+    w.stepMax = stepMax;
+    w.k = ClampZ(q, FindZ(q, p.z));
+    w.step = d.z >= 0 ? 1 : -1;
+    w.distance = kInfinity;
+    return false;
   }
+  static VGA_HD bool DistOutStep(const P &q, DistOutWalk &w) {
+    w.distance = DistanceToOutSegment(q, w.k, w.p, w.d);
+    if (w.distance >= 0 && w.distance < kInfinity) return true;
+    if (w.step > 0 ? (Z(q, w.k) - w.p.z > w.distance) : (w.p.z - Z(q, w.k) > w.distance)) return true;
+    w.k += w.step;
+    return w.k < 0 || w.k >= NSeg(q);
+  }
+  static VGA_HD double DistOutEnd(const P &q, DistOutWalk &w) {
+    double distance = w.distance;
+    DistanceToEndcaps<true>(q, w.p, w.d, distance);
+    return distance < w.stepMax ? distance : w.stepMax;
+  }
+  static VGA_HD double DistanceToOut(const P &q, V3 p, V3 d, double stepMax) {
+    DistOutWalk w;
+    if (!DistOutBegin(q, p, d, stepMax, w))
+      while (!DistOutStep(q, w)) {
+      }
+    return DistOutEnd(q, w);
+  }
+  struct DistOutOps {
+    using State = DistOutWalk;
+    static VGA_HD bool Begin(const P &q, V3 p, V3 d, double sm, State &w) { return DistOutBegin(q, p, d, sm, w); }
+    static VGA_HD bool Step(const P &q, State &w) { return DistOutStep(q, w); }
+    static VGA_HD double End(const P &q, State &w) { return DistOutEnd(q, w); }
+  };
 
   // ScalarSafetyKernel, :915-944 (both SafetyToIn and SafetyToOut)
   static VGA_HD double Safety(const P &q, V3 p) {

@@ -165,11 +165,13 @@ constexpr int ClassOf(int type) {
 // takes it as given, the compute-heavy classes at most 2.
 constexpr int ClassIT(int c, int it) { return c == 0 ? it : (it < 2 ? it : 2); }
 
-// DistanceToIn tasks of the Polyhedron in the staged class run with lane
-// refill (vg_walk.hpp) over tiles kWalkTileMul times longer, so that each
-// wave refills its lanes from 4 * 64 * IT queries.
+// DistanceToIn and DistanceToOut tasks of the Polyhedron in the staged class
+// run with lane refill (vg_walk.hpp) over tiles kWalkTileMul times longer,
+// so that each wave refills its lanes from 4 * 64 * IT queries.
 constexpr int kWalkTileMul = 4;
-constexpr bool WalkTask(int type, int query) { return query == kQDistIn && type == VG_POLYHEDRON; }
+constexpr bool WalkTask(int type, int query) {
+  return type == VG_POLYHEDRON && (query == kQDistIn || query == kQDistOut || query == kQPlacedDistOut);
+}
 constexpr uint64_t TileItems(int c, int it, int type, int query) {
   return (uint64_t)kBatchBlock * ClassIT(c, it) * ((c == kStagedClass && WalkTask(type, query)) ? kWalkTileMul : 1);
 }
@@ -260,10 +262,11 @@ __device__ __forceinline__ double LoadOne(const TaskDev &t, const void *a, uint6
   return t.in_type == VG_F32 ? (double)static_cast<const float *>(a)[i] : static_cast<const double *>(a)[i];
 }
 
-// DistanceToIn of items [first, first + count) of a walk task with lane
-// refill: each of the workgroup's waves takes a quarter of the range.
-// Called by every thread of the workgroup.
-template <class S>
+// Items [first, first + count) of a walk task with lane refill: each of the
+// workgroup's waves takes a quarter of the range.  Local says the inputs are
+// in the shape's frame already (DistanceToOut).  Called by every thread of
+// the workgroup.
+template <class S, class Ops, bool Local>
 __device__ __forceinline__ void RunTileWalk(const TaskDev &t, const double *blk, uint64_t first, uint64_t count) {
   constexpr int kWaves = kBatchBlock / 64;
   const uint64_t per = (count + kWaves - 1) / kWaves;
@@ -272,11 +275,15 @@ __device__ __forceinline__ void RunTileWalk(const TaskDev &t, const double *blk,
   const uint64_t begin = b < lim ? b : lim;
   const uint64_t end = begin + per < lim ? begin + per : lim;
   double *out = static_cast<double *>(t.out);
-  WalkRefill<S>(
+  WalkRefill<Ops>(
       ParamView<S>::Get(blk), begin, end,
       [&](uint64_t i, V3 &p, V3 &d, double &sm) {
-        p = XfPointRt(t.xf, t.xfkind, V3{LoadOne(t, t.x, i), LoadOne(t, t.y, i), LoadOne(t, t.z, i)});
-        d = XfDirRt(t.xf, t.xfkind, V3{LoadOne(t, t.dx, i), LoadOne(t, t.dy, i), LoadOne(t, t.dz, i)});
+        p = V3{LoadOne(t, t.x, i), LoadOne(t, t.y, i), LoadOne(t, t.z, i)};
+        d = V3{LoadOne(t, t.dx, i), LoadOne(t, t.dy, i), LoadOne(t, t.dz, i)};
+        if (!Local) {
+          p = XfPointRt(t.xf, t.xfkind, p);
+          d = XfDirRt(t.xf, t.xfkind, d);
+        }
         sm = t.sm ? t.sm[i] : kInfinity;
       },
       [&](uint64_t i, double r) { out[i] = r; });
@@ -326,8 +333,13 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(Cla
       Stage(stage, params + t.param_off, t.param_len);
       __syncthreads();
       if (WalkTask(t.type, t.query)) {  // uniform over the workgroup
-        const uint64_t per = TileItems(CLS, IT, t.type, t.query);
-        RunTileWalk<PolyhedronS>(t, stage, (tile - t.tile_begin) * per, per);
+        const uint64_t per = TileItems(CLS, IT, t.type, t.query), first = (tile - t.tile_begin) * per;
+        if (t.query == kQDistIn)
+          RunTileWalk<PolyhedronS, PolyhedronS::DistInOps, false>(t, stage, first, per);
+        else if (t.query == kQDistOut)
+          RunTileWalk<PolyhedronS, PolyhedronS::DistOutOps, true>(t, stage, first, per);
+        else
+          RunTileWalk<PolyhedronS, PolyhedronS::DistOutOps, false>(t, stage, first, per);
       } else if (i0 < t.n) {
         RunTask<IT, CLS>(t, stage, i0);
       }

@@ -280,14 +280,15 @@ struct HasWalk<S, decltype((void)S::kHasWalk)> {
   static constexpr bool value = S::kHasWalk;
 };
 
-// DistanceToIn with lane refill (vg_walk.hpp) for shapes whose DistanceToIn
-// is a walk of variable length: each wave owns kWalkPerLane * 64 consecutive
-// queries.
-constexpr int kWalkPerLane = 8;
-
-template <class S, int XK, bool Stage>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWalkWaves))) void WalkDistInKernel(const typename S::P Pg, const Xf X, const QArgs a,
-                                                           int stage_len) {
+// Distances with lane refill (vg_walk.hpp) for shapes whose DistanceToIn /
+// DistanceToOut is a walk of variable length: each wave owns per_lane * 64
+// consecutive queries.  Ops is S::DistInOps (points and directions
+// transformed by XK) or S::DistOutOps (XK the identity for local inputs, the
+// placement for PlacedDistanceToOut); NextIndex writes the DistanceToOut
+// node index.
+template <class S, class Ops, int XK, bool Stage, bool NextIndex>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWalkWaves))) void WalkKernel(
+    const typename S::P Pg, const Xf X, const QArgs a, int stage_len, int per_lane) {
   typename S::P P = Pg;
   if constexpr (Stage) {
     extern __shared__ __align__(16) double stage[];
@@ -295,18 +296,43 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWalkWav
     __syncthreads();
     P = typename S::P{stage};
   }
-  constexpr uint64_t kChunk = 64 * kWalkPerLane;
-  const uint64_t begin = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / 64 * kChunk;
-  const uint64_t end = begin + kChunk < a.n ? begin + kChunk : a.n;
+  const uint64_t chunk = 64 * (uint64_t)per_lane;
+  const uint64_t begin = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / 64 * chunk;
+  const uint64_t end = begin + chunk < a.n ? begin + chunk : a.n;
   double *out = reinterpret_cast<double *>(a.out);
-  WalkRefill<S>(
+  WalkRefill<Ops>(
       P, begin, end,
       [&](uint64_t i, V3 &p, V3 &d, double &sm) {
         p = XfPoint<XK>(X, V3{a.x[i], a.y[i], a.z[i]});
         d = XfDir<XK>(X, V3{a.dx[i], a.dy[i], a.dz[i]});
         sm = a.sm ? a.sm[i] : kInfinity;
       },
-      [&](uint64_t i, double r) { out[i] = r; });
+      [&](uint64_t i, double r) {
+        out[i] = r;
+        // ShapeImplementationHelper.h:677-692 (DistanceToOut with node index)
+        if (NextIndex && a.next) a.next[i] = (r < (a.sm ? a.sm[i] : kInfinity)) ? -1 : -2;
+      });
+}
+
+// Queries per lane of a walk launch: up to 8, fewer for small baskets so
+// that the grid still holds about 4096 waves.
+int WalkPerLane(uint64_t n) {
+  const uint64_t q = n / (64ull * 4096ull);
+  return q < 1 ? 1 : (q > 8 ? 8 : (int)q);
+}
+
+template <class S, class Ops, int XK, bool NextIndex = false>
+int LaunchWalk(const typename S::P &P, const Xf &X, const QArgs &a, int stage_len, hipStream_t s) {
+  const int per_lane = WalkPerLane(a.n);
+  const uint64_t waves = (a.n + 64ull * per_lane - 1) / (64ull * per_lane);
+  const uint64_t blocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
+  if (stage_len > 0 && stage_len <= kMaxStageDoubles)
+    hipLaunchKernelGGL((WalkKernel<S, Ops, XK, true, NextIndex>), dim3((unsigned)blocks), dim3(kBlock),
+                       stage_len * sizeof(double), s, P, X, a, stage_len, per_lane);
+  else
+    hipLaunchKernelGGL((WalkKernel<S, Ops, XK, false, NextIndex>), dim3((unsigned)blocks), dim3(kBlock), 0, s, P, X, a, 0,
+                       per_lane);
+  return CheckHip(hipGetLastError(), "shape query launch");
 }
 
 // Plain FP64 form (the benchmark denominator): one thread per query, the
@@ -352,18 +378,11 @@ int LaunchQ(const typename S::P &P, const Xf &X, const QArgs &a, int stage_len, 
   bool al = Aligned16(a.x) && Aligned16(a.y) && Aligned16(a.z) && Aligned16(a.out);
   if (NeedsDir(Q)) al = al && Aligned16(a.dx) && Aligned16(a.dy) && Aligned16(a.dz);
   if (NeedsStep(Q) && a.sm) al = al && Aligned16(a.sm);
-  if constexpr (Q == kQDistIn && HasWalk<S>::value) {
-    const uint64_t waves = (a.n + 64 * kWalkPerLane - 1) / (64 * kWalkPerLane);
-    const uint64_t blocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
-    if constexpr (ParamView<S>::kIndirect) {
-      if (stage_len > 0 && stage_len <= kMaxStageDoubles) {
-        hipLaunchKernelGGL((WalkDistInKernel<S, XK, true>), dim3((unsigned)blocks), dim3(kBlock),
-                           stage_len * sizeof(double), s, P, X, a, stage_len);
-        return CheckHip(hipGetLastError(), "shape query launch");
-      }
-    }
-    hipLaunchKernelGGL((WalkDistInKernel<S, XK, false>), dim3((unsigned)blocks), dim3(kBlock), 0, s, P, X, a, 0);
-    return CheckHip(hipGetLastError(), "shape query launch");
+  if constexpr (HasWalk<S>::value) {
+    if constexpr (Q == kQDistIn) return LaunchWalk<S, typename S::DistInOps, XK>(P, X, a, stage_len, s);
+    if constexpr (Q == kQDistOut)
+      return LaunchWalk<S, typename S::DistOutOps, kXfIdentity, true>(P, X, a, stage_len, s);
+    if constexpr (Q == kQPlacedDistOut) return LaunchWalk<S, typename S::DistOutOps, XK>(P, X, a, stage_len, s);
   }
   if constexpr (Q == kQDistIn && HasQuick<S>::value) {
     const uint64_t blocks = (a.n + kBlock - 1) / kBlock;

@@ -1,8 +1,8 @@
-// Lane refill for DistanceToIn walks on gfx950 (64-lane waves).
+// Lane refill for distance walks on gfx950 (64-lane waves).
 //
-// Shapes whose DistanceToIn is a walk of data-dependent length (the
-// Polyhedron's z segments) expose it as S::DistInBegin /
-// DistInStep / DistInEnd over a per-lane S::DistInWalk state.  In the
+// Shapes whose DistanceToIn or DistanceToOut is a walk of data-dependent
+// length (the Polyhedron's z segments) expose it as an ops type with
+// Begin / Step / End over a per-lane State (S::DistInOps, S::DistOutOps).  In the
 // one-query-per-lane form a wave runs as long as its longest walk while the
 // lanes whose walks ended idle.  Here a wave owns a range of queries: a lane
 // without a query takes the next unassigned one (ballot + mbcnt, no atomics)
@@ -29,11 +29,10 @@ namespace vga {
 constexpr int kRefillMin = 32;
 
 // Must be called by all 64 lanes of a wave with the same [begin, end).
-// load(i, p, d, sm) reads query i (local frame); store(i, r) writes its result.
-template <class S, class Load, class Store>
-__device__ __forceinline__ void WalkRefill(const typename S::P &P, uint64_t begin, uint64_t end, Load load,
-                                           Store store) {
-  typename S::DistInWalk w;
+// load(i, p, d, sm) reads query i (shape frame); store(i, r) writes its result.
+template <class Ops, class PT, class Load, class Store>
+__device__ __forceinline__ void WalkRefill(const PT &P, uint64_t begin, uint64_t end, Load load, Store store) {
+  typename Ops::State w;
   uint64_t next = begin, qi = 0;
   bool active = false;
   for (;;) {
@@ -49,8 +48,8 @@ __device__ __forceinline__ void WalkRefill(const typename S::P &P, uint64_t begi
           double sm;
           load(i, p, d, sm);
           qi = i;
-          if (S::DistInBegin(P, p, d, sm, w))
-            store(i, S::DistInEnd(P, w));
+          if (Ops::Begin(P, p, d, sm, w))
+            store(i, Ops::End(P, w));
           else
             active = true;
         }
@@ -58,8 +57,8 @@ __device__ __forceinline__ void WalkRefill(const typename S::P &P, uint64_t begi
       next += (uint64_t)__popcll(need);
     }
     if (__ballot(active) == 0) break;
-    if (active && S::DistInStep(P, w)) {
-      store(qi, S::DistInEnd(P, w));
+    if (active && Ops::Step(P, w)) {
+      store(qi, Ops::End(P, w));
       active = false;
     }
   }
