@@ -1,5 +1,5 @@
 """Known answers of the reference's unit tests (TestBox.cpp, TestTube.cpp, TestPolycone.cpp,
-TestPolyhedra.cpp, TestOrb.cpp, TestTrd.cpp),
+TestPolyhedra.cpp, TestOrb.cpp, TestTrd.cpp, TestTrap.cpp),
 transcribed as data in tests/golden/known_answers.json, checked on the
 oracle here and on the GPU in test_gpu_shapes.py."""
 import json

@@ -310,7 +310,9 @@ struct PolyhedronS {
     return true;
   }
 
-  // ScalarContainsKernel, :728-770
+  // ScalarContainsKernel, :728-770 (a form that evaluates every test on
+  // every lane and combines the results measured slower: whole waves of
+  // points outside the bounding tube skip the plane loops here).
   static VGA_HD bool Contains(const P &q, V3 p) {
     if (!TubeS::Contains(Tube(q), BoundsPoint(q, p))) return false;
     const int k = FindZ(q, p.z);
@@ -427,7 +429,7 @@ struct PolyhedronS {
   // here skips ahead over such segments.  DistanceToIn runs the walk on one
   // lane; the refill kernel interleaves the walks of a wave's lanes so that
   // every DistInStep runs on as many lanes as possible.
-  static constexpr bool kHasWalk = true;
+  static constexpr bool kWalkIn = true, kWalkOut = true;
   struct DistInWalk {
     V3 p, d;
     double stepMax, distance;

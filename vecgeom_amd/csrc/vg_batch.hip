@@ -165,9 +165,10 @@ constexpr int ClassOf(int type) {
 // takes it as given, the compute-heavy classes at most 2.
 constexpr int ClassIT(int c, int it) { return c == 0 ? it : (it < 2 ? it : 2); }
 
-// DistanceToIn and DistanceToOut tasks of the Polyhedron in the staged class
-// run with lane refill (vg_walk.hpp) over tiles kWalkTileMul times longer,
-// so that each wave refills its lanes from 4 * 64 * IT queries.
+// The Polyhedron's distance tasks in the staged class run with lane refill
+// (vg_walk.hpp) over tiles kWalkTileMul times longer, so that each wave
+// refills its lanes from 4 * 64 * IT queries.  (The Polycone's measured
+// slower as walks, DistanceToIn and DistanceToOut alike.)
 constexpr int kWalkTileMul = 4;
 constexpr bool WalkTask(int type, int query) {
   return type == VG_POLYHEDRON && (query == kQDistIn || query == kQDistOut || query == kQPlacedDistOut);
@@ -334,12 +335,13 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(Cla
       __syncthreads();
       if (WalkTask(t.type, t.query)) {  // uniform over the workgroup
         const uint64_t per = TileItems(CLS, IT, t.type, t.query), first = (tile - t.tile_begin) * per;
-        if (t.query == kQDistIn)
+        if (t.query == kQDistIn) {
           RunTileWalk<PolyhedronS, PolyhedronS::DistInOps, false>(t, stage, first, per);
-        else if (t.query == kQDistOut)
+        } else if (t.query == kQDistOut) {
           RunTileWalk<PolyhedronS, PolyhedronS::DistOutOps, true>(t, stage, first, per);
-        else
+        } else {
           RunTileWalk<PolyhedronS, PolyhedronS::DistOutOps, false>(t, stage, first, per);
+        }
       } else if (i0 < t.n) {
         RunTask<IT, CLS>(t, stage, i0);
       }

@@ -271,13 +271,14 @@ __global__ __launch_bounds__(kBlock) void CompactDistInKernel(const typename S::
   }
 }
 
+// Walk flags of a shape: S::kWalkIn / S::kWalkOut when it declares them.
 template <class S, class = void>
-struct HasWalk {
-  static constexpr bool value = false;
+struct Walks {
+  static constexpr bool in = false, out = false;
 };
 template <class S>
-struct HasWalk<S, decltype((void)S::kHasWalk)> {
-  static constexpr bool value = S::kHasWalk;
+struct Walks<S, decltype((void)S::kWalkOut)> {
+  static constexpr bool in = S::kWalkIn, out = S::kWalkOut;
 };
 
 // Distances with lane refill (vg_walk.hpp) for shapes whose DistanceToIn /
@@ -378,12 +379,11 @@ int LaunchQ(const typename S::P &P, const Xf &X, const QArgs &a, int stage_len, 
   bool al = Aligned16(a.x) && Aligned16(a.y) && Aligned16(a.z) && Aligned16(a.out);
   if (NeedsDir(Q)) al = al && Aligned16(a.dx) && Aligned16(a.dy) && Aligned16(a.dz);
   if (NeedsStep(Q) && a.sm) al = al && Aligned16(a.sm);
-  if constexpr (HasWalk<S>::value) {
-    if constexpr (Q == kQDistIn) return LaunchWalk<S, typename S::DistInOps, XK>(P, X, a, stage_len, s);
-    if constexpr (Q == kQDistOut)
-      return LaunchWalk<S, typename S::DistOutOps, kXfIdentity, true>(P, X, a, stage_len, s);
-    if constexpr (Q == kQPlacedDistOut) return LaunchWalk<S, typename S::DistOutOps, XK>(P, X, a, stage_len, s);
-  }
+  if constexpr (Q == kQDistIn && Walks<S>::in) return LaunchWalk<S, typename S::DistInOps, XK>(P, X, a, stage_len, s);
+  if constexpr (Q == kQDistOut && Walks<S>::out)
+    return LaunchWalk<S, typename S::DistOutOps, kXfIdentity, true>(P, X, a, stage_len, s);
+  if constexpr (Q == kQPlacedDistOut && Walks<S>::out)
+    return LaunchWalk<S, typename S::DistOutOps, XK>(P, X, a, stage_len, s);
   if constexpr (Q == kQDistIn && HasQuick<S>::value) {
     const uint64_t blocks = (a.n + kBlock - 1) / kBlock;
     if constexpr (ParamView<S>::kIndirect) {
