@@ -1,5 +1,6 @@
-"""GPU tests of the Polyhedron's DistanceToIn walk with lane refill
-(WalkDistInKernel for single launches, the staged batch class for batches):
+"""GPU tests of the Polyhedron's DistanceToIn and DistanceToOut walks with
+lane refill (WalkKernel for single launches, the staged batch class for
+batches):
 bit-exact against the oracle for basket sizes around the 64-lane and
 512-query chunk boundaries, for polyhedra whose parameter block is staged in
 LDS and one that is read from global memory, in FP64 and with FP32-storage
@@ -82,4 +83,43 @@ def test_walk_batched_matches_oracle(gpu, case, oracle, dtype):
             want = o.query(po.Q_DIST_IN, pf, df, np.full(n, 1e30))
             ok = same(t.out.cpu().numpy(), want)
             assert ok.all(), f"{name} {dtype} it={it} n={n}: {int((~ok).sum())} lanes differ"
+    b.free()
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_walk_out_matches_oracle(gpu, case, oracle, dtype):
+    """DistanceToOut (local frame) and PlacedDistanceToOut from points inside
+    the shape, single launches and batched."""
+    name, u, xf, geom, pid, inp = case
+    o = oracle.placed(u.type, u.params, xf.array())
+    inside = np.flatnonzero(o.query(po.Q_CONTAINS, inp["p"]).astype(bool))
+    lp, ld = inp["lp"][:, inside], inp["ld"][:, inside]
+    p, d = inp["p"][:, inside], inp["d"][:, inside]
+    sizes = [n for n in SIZES if n <= inside.size] + [inside.size]
+    tasks, want = [], []
+    for n in sizes:
+        sm = np.full(n, 1e30)
+        for q, pp, dd, oq in ((B.DISTANCE_TO_OUT, lp, ld, po.Q_DIST_OUT), (B.PLACED_DISTANCE_TO_OUT, p, d,
+                                                                           po.Q_PLACED_DIST_OUT)):
+            pt, dt = dev(pp[:, :n], gpu, dtype), dev(dd[:, :n], gpu, dtype)
+            tasks.append(B.Task(pid, q, pt, dt, torch.from_numpy(sm).to(gpu)))
+            pf = np.stack([a.cpu().numpy().astype(np.float64) for a in pt])
+            df = np.stack([a.cpu().numpy().astype(np.float64) for a in dt])
+            want.append(o.query(oq, pf, df, sm))
+    if dtype == torch.float64:
+        for t, w in zip(tasks, want):
+            fn = Q.distance_to_out if t.query == B.DISTANCE_TO_OUT else Q.placed_distance_to_out
+            got = fn(geom, pid, t.p, t.d, t.step_max).cpu().numpy()
+            ok = same(got, w)
+            assert ok.all(), f"{name} single q={t.query} n={t.n}: {int((~ok).sum())} lanes differ"
+    b = B.Batch(geom, tasks)
+    for it in (1, 2):
+        b.set_launch(it, 0)
+        for t in tasks:
+            t.out.fill_(7)
+        b.run()
+        torch.cuda.synchronize()
+        for t, w in zip(tasks, want):
+            ok = same(t.out.cpu().numpy(), w)
+            assert ok.all(), f"{name} {dtype} it={it} q={t.query} n={t.n}: {int((~ok).sum())} lanes differ"
     b.free()

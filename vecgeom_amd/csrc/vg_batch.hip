@@ -418,6 +418,12 @@ int QueryCode(int32_t q) {
   return -1;
 }
 
+// Relative cost of a query kind (distances walk and intersect, safeties
+// walk, classifications only test).
+int QueryWeight(int q) {
+  return q == kQDistIn ? 3 : (q == kQContains || q == kQInside) ? 1 : 2;
+}
+
 int TaskClass(const TaskDev &t) {
   const int c = ClassOf(t.type);
   return (c == kStagedClass && t.param_len > kMaxStageDoubles) ? kStagedClass + 1 : c;
@@ -588,8 +594,13 @@ int vg_batch_create(const vg_geometry *g, const vg_query_task *tasks, int32_t nt
     b->items += t.n;
     b->host.push_back(t);
   }
-  std::stable_sort(b->host.begin(), b->host.end(),
-                   [](const TaskDev &a, const TaskDev &c) { return TaskClass(a) < TaskClass(c); });
+  // By class, and within a class the costlier queries first: tiles are
+  // dispatched roughly in order, so a launch ends on short tiles instead of
+  // on a few long ones (neutral on the zoo step, whose classes are balanced).
+  std::stable_sort(b->host.begin(), b->host.end(), [](const TaskDev &a, const TaskDev &c) {
+    const int ka = TaskClass(a), kc = TaskClass(c);
+    return ka != kc ? ka < kc : QueryWeight(a.query) > QueryWeight(c.query);
+  });
   for (int c = 0; c <= kClasses; ++c) {
     int k = 0;
     while (k < (int)b->host.size() && TaskClass(b->host[k]) < c) ++k;
