@@ -253,13 +253,14 @@ int vg_query(const vg_geometry *g, const vg_query_task *task, void *stream);
 
 /*
  * A batch: a fixed table of tasks (any mix of placed volumes, queries, basket
- * sizes and input types) run by ONE kernel launch, each workgroup tile
- * belonging to one task.  This replaces the per-shape, per-query launches of
+ * sizes and input types) run by one kernel launch per shape weight class (at
+ * most five, issued on forked streams and joined back to the caller's), each
+ * workgroup tile belonging to one task.  This replaces the per-shape, per-query launches of
  * Benchmarker::Run*Cuda (source/benchmarking/Benchmarker.cu:86-367).  The
  * arrays named in the tasks must stay valid while the batch is used; the
  * launch reads whatever they hold when it runs, so a caller refills inputs
- * and runs again (also from a captured hipGraph: vg_batch_run enqueues one
- * kernel and nothing else).
+ * and runs again (also from a captured hipGraph: vg_batch_run enqueues its
+ * class kernels and the fork/join events, nothing else).
  */
 typedef struct vg_batch vg_batch;
 int vg_batch_create(const vg_geometry *g, const vg_query_task *tasks, int32_t ntasks, vg_batch **out);
