@@ -147,8 +147,13 @@ struct PolyhedronS {
     const double rho = Sqrt(p.x * p.x + p.y * p.y);
     const double dr = rho < r[7] ? r[7] - rho : (rho > r[8] ? rho - r[8] : 0.);
     const double scale = Abs(p.x) + Abs(p.y) + Abs(p.z) + r[8] + Abs(zlo) + Abs(zhi);
-    const double lb = Sqrt(dz * dz + dr * dr) - (1e-6 + 1e-8 * scale);
-    return lb > 0 && lb * lb >= safety2;
+    const double pad = 1e-6 + 1e-8 * scale;
+    // The slab distance sqrt(d2) exceeds sqrt(safety2) by more than pad when
+    // d2 >= safety2 + pad (safety2 + 1) + pad^2, since 2 sqrt(s) <= s + 1;
+    // the last factor covers the rounding of the right-hand side.  (No
+    // square root; it prunes a pad-wide sliver less than the exact bound.)
+    const double d2 = dz * dz + dr * dr;
+    return d2 >= (safety2 + pad * (safety2 + 1.) + pad * pad) * (1. + 1e-12);
   }
 
   // DistanceToLineSegmentSquared<kScalar>, GenericKernels.h:118-147
