@@ -97,13 +97,26 @@ struct PolyhedronS {
   // the slab gets kInfinity from every quadrilateral of the segment.  1 when
   // the ray may hit the outer or phi quadrilaterals but stays more than kPad
   // outside radius rInHi within the slab (no inner hit possible), 2 otherwise.
-  static VGA_HD int RayNearSegment(const P &q, int k, V3 p, V3 v) {
+  // The ray's constants of RayNearSegment, computed once per query.
+  struct RayPre {
+    double invz, tc;  // 1 / v.z; the parameter of closest approach to the z axis
+    bool radial;      // v has a component across the z axis
+  };
+  static VGA_HD RayPre MakeRayPre(V3 p, V3 v) {
+    RayPre r;
+    r.invz = v.z != 0 ? 1. / v.z : 0.;
+    const double a = v.x * v.x + v.y * v.y, b = p.x * v.x + p.y * v.y;
+    r.radial = a > 0;
+    r.tc = r.radial ? -b / a : 0.;
+    return r;
+  }
+  static VGA_HD int RayNearSegment(const P &q, int k, V3 p, V3 v, const RayPre &pre) {
     constexpr double kPad = 1e-6;
     const double *r = SegRec(q, k);
     const double zlo = Z(q, k) - kPad, zhi = Z(q, k + 1) + kPad;
     double t0 = 0., t1 = kInfinity;
     if (v.z != 0) {
-      const double inv = 1. / v.z;
+      const double inv = pre.invz;
       double ta = (zlo - p.z) * inv, tb = (zhi - p.z) * inv;
       if (ta > tb) {
         const double t = ta;
@@ -116,8 +129,7 @@ struct PolyhedronS {
     } else if (p.z < zlo || p.z > zhi) {
       return false;
     }
-    const double a = v.x * v.x + v.y * v.y, b = p.x * v.x + p.y * v.y;
-    double t = a > 0 ? -b / a : t0;
+    double t = pre.radial ? pre.tc : t0;
     t = t < t0 ? t0 : (t > t1 ? t1 : t);
     const double x = p.x + t * v.x, y = p.y + t * v.y;
     const double rho2 = x * x + y * y;  // smallest rho^2 within the slab
@@ -140,11 +152,11 @@ struct PolyhedronS {
   // squared distance to the quadrilateral (point inside its side planes) or
   // to one of its edges, never below the squared distance to the slab by
   // more than rounding; pad covers that rounding with a wide margin.
-  static VGA_HD bool SegmentSafetyAtLeast(const P &q, int k, V3 p, double safety2) {
+  // rho: Sqrt(p.x * p.x + p.y * p.y), computed once per query.
+  static VGA_HD bool SegmentSafetyAtLeast(const P &q, int k, V3 p, double rho, double safety2) {
     const double *r = SegRec(q, k);
     const double zlo = Z(q, k), zhi = Z(q, k + 1);
     const double dz = p.z < zlo ? zlo - p.z : (p.z > zhi ? p.z - zhi : 0.);
-    const double rho = Sqrt(p.x * p.x + p.y * p.y);
     const double dr = rho < r[7] ? r[7] - rho : (rho > r[8] ? rho - r[8] : 0.);
     const double scale = Abs(p.x) + Abs(p.y) + Abs(p.z) + r[8] + Abs(zlo) + Abs(zhi);
     const double pad = 1e-6 + 1e-8 * scale;
@@ -438,6 +450,7 @@ struct PolyhedronS {
   struct DistInWalk {
     V3 p, d;
     double stepMax, distance;
+    RayPre pre;
     int k, step, near;
     bool final;
   };
@@ -445,11 +458,11 @@ struct PolyhedronS {
   // when there is none.
   static VGA_HD bool DistInAdvance(const P &q, DistInWalk &w) {
     const int n = NSeg(q);
-    int near = RayNearSegment(q, w.k, w.p, w.d);
+    int near = RayNearSegment(q, w.k, w.p, w.d, w.pre);
     while (!near) {
       w.k += w.step;
       if (w.k < 0 || w.k >= n) return false;
-      near = RayNearSegment(q, w.k, w.p, w.d);
+      near = RayNearSegment(q, w.k, w.p, w.d, w.pre);
     }
     w.near = near;
     return true;
@@ -474,6 +487,7 @@ struct PolyhedronS {
     w.final = false;
     w.step = d.z >= 0 ? 1 : -1;
     w.distance = kInfinity;
+    w.pre = MakeRayPre(p, d);
     return !DistInAdvance(q, w);
   }
   // True when the walk is over.
@@ -556,15 +570,16 @@ struct PolyhedronS {
     double safety = kInfinity;
     const int k = ClampZ(q, FindZ(q, p.z));
     const int phiIndex = FindPhi(q, p);
+    const double rho = Sqrt(p.x * p.x + p.y * p.y);
     // (Skipping ahead over pruned segments per lane, as DistanceToIn does,
     // measured slower here: the segment computation is short.)
     for (int z = k; z < n;) {
-      if (!SegmentSafetyAtLeast(q, z, p, safety)) safety = Min(safety, SafetySegmentSquared(q, z, phiIndex, p));
+      if (!SegmentSafetyAtLeast(q, z, p, rho, safety)) safety = Min(safety, SafetySegmentSquared(q, z, phiIndex, p));
       ++z;
       if (Z(q, z) - p.z > safety) break;
     }
     for (int z = k - 1; z >= 0; --z) {
-      if (!SegmentSafetyAtLeast(q, z, p, safety)) safety = Min(safety, SafetySegmentSquared(q, z, phiIndex, p));
+      if (!SegmentSafetyAtLeast(q, z, p, rho, safety)) safety = Min(safety, SafetySegmentSquared(q, z, phiIndex, p));
       if (p.z - Z(q, z) > safety) break;
     }
     SafetyToEndcapsSquared(q, p, safety);
