@@ -74,7 +74,8 @@ def test_gpu_known_answers(gpu):
         geom, ids, _ = G.single_shape_geometry([G.UnplacedVolume(row["type"], row["params"])], [G.Transformation3D()])
         p = tuple(torch.tensor([v], dtype=torch.float64, device=gpu) for v in row["p"])
         d = None if row["d"] is None else tuple(torch.tensor([v], dtype=torch.float64, device=gpu) for v in row["d"])
-        fn = {"inside": lambda: Q.inside(geom, ids[0], p), "dist_in": lambda: Q.distance_to_in(geom, ids[0], p, d),
+        fn = {"contains": lambda: Q.contains(geom, ids[0], p), "inside": lambda: Q.inside(geom, ids[0], p),
+              "dist_in": lambda: Q.distance_to_in(geom, ids[0], p, d),
               "dist_out": lambda: Q.distance_to_out(geom, ids[0], p, d),
               "safe_in": lambda: Q.safety_to_in(geom, ids[0], p),
               "safe_out": lambda: Q.safety_to_out(geom, ids[0], p)}[row["query"]]
