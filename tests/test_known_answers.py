@@ -1,6 +1,6 @@
 """Known answers of the reference's unit tests (TestBox.cpp, TestTube.cpp, TestPolycone.cpp,
 TestPolyhedra.cpp, TestOrb.cpp, TestTrd.cpp, TestTrap.cpp,
-TestCons.cpp, TestSphere.cpp, TestVecGeomPolycone.cpp),
+TestCons.cpp, TestSphere.cpp, TestVecGeomPolycone.cpp, TestBoxlikeTrapezoid.cpp),
 transcribed as data in tests/golden/known_answers.json, checked on the
 oracle here and on the GPU in test_gpu_shapes.py."""
 import json
