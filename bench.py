@@ -32,7 +32,34 @@ BYTES = {"contains": 25, "inside": 28, "dist_in": 64, "dist_out": 64, "safe_in":
 BYTES_F32 = {"contains": 13, "inside": 16, "dist_in": 40, "dist_out": 40, "safe_in": 20, "safe_out": 20}
 
 
+# (shape, query) pairs left out of the step: the reference's Sphere
+# DistanceToIn has an empty body (volumes/kernel/SphereImplementation.h:88-94)
+# and returns kInfinity without reading its inputs, so timing it measures
+# nothing but a store.
+EXCLUDED = {("Sphere", "dist_in")}
+
 PROFILE = os.path.join(ROOT, "profiles", "shape_zoo_fp64_kernels.md")
+BATCH_PROFILE = os.path.join(ROOT, "profiles", "shape_zoo_batched_kernels.md")
+
+
+def ClassIT(c, it):
+    """Items per lane of weight class c for a batch launched with `it`."""
+    return it if c == 0 else min(it, 2)
+
+
+def profiled_class_traffic(cls, items):
+    """HBM bytes per launch of a batch class kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (bytes per query, scaled by a
+    device copy of known size) times its queries; None if not profiled."""
+    try:
+        with open(BATCH_PROFILE) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) >= 6 and cells[0] == f"class {cls}":
+                    return int(round(float(cells[5]) * items))
+    except (OSError, ValueError):
+        pass
+    return None
 KERNEL_SHAPE = {"Parallelepiped": "Para", "Paraboloid": "Parab", "Trapezoid": "Trap"}
 
 
@@ -73,7 +100,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--points", type=int, default=1 << 22)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="target CPU-baseline duration (shapes)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-FP64-form denominator")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched and FP32-storage forms")
@@ -135,7 +162,11 @@ class Workload:
             }
             it["count"] = {"contains": n, "inside": n, "dist_in": n_out, "safe_in": n_out, "dist_out": n_in,
                            "safe_out": n_in}
+            for q in list(it["count"]):
+                if (name, q) in EXCLUDED:
+                    del it["count"][q]
             self.items.append(it)
+        self.names = [it["name"] for it in self.items]
         self.queries_per_step = sum(sum(it["count"].values()) for it in self.items)
         self.bytes_per_step = sum(c * BYTES[q] for it in self.items for q, c in it["count"].items())
         self.bytes_per_step_f32 = sum(c * BYTES_F32[q] for it in self.items for q, c in it["count"].items())
@@ -152,15 +183,20 @@ class Workload:
         for it in self.items:
             o, pid = it["out"], it["pid"]
             p, pout, dout, pin, din = cv(it["p"]), cv(it["pout"]), cv(it["dout"]), cv(it["pin"]), cv(it["din"])
-            out += [B.Task(pid, B.CONTAINS, p, out=o["contains"]), B.Task(pid, B.INSIDE, p, out=o["inside"]),
-                    B.Task(pid, B.DISTANCE_TO_IN, pout, dout, it["smout"], out=o["dist_in"]),
-                    B.Task(pid, B.SAFETY_TO_IN, pout, out=o["safe_in"]),
-                    B.Task(pid, B.DISTANCE_TO_OUT, pin, din, it["smin"], out=o["dist_out"]),
-                    B.Task(pid, B.SAFETY_TO_OUT, pin, out=o["safe_out"])]
+            tasks = {"contains": B.Task(pid, B.CONTAINS, p, out=o["contains"]),
+                     "inside": B.Task(pid, B.INSIDE, p, out=o["inside"]),
+                     "dist_in": B.Task(pid, B.DISTANCE_TO_IN, pout, dout, it["smout"], out=o["dist_in"]),
+                     "safe_in": B.Task(pid, B.SAFETY_TO_IN, pout, out=o["safe_in"]),
+                     "dist_out": B.Task(pid, B.DISTANCE_TO_OUT, pin, din, it["smin"], out=o["dist_out"]),
+                     "safe_out": B.Task(pid, B.SAFETY_TO_OUT, pin, out=o["safe_out"])}
+            out += [t for q, t in tasks.items() if q in it["count"]]
         return out
 
     def launches(self):
         """(key, callable) for every launch of one step."""
+        return [(key, fn) for key, fn in self._launches() if key[1] in self.items[self.names.index(key[0])]["count"]]
+
+    def _launches(self):
         Q, g = self.Q, self.geom
         out = []
         for it in self.items:
@@ -206,60 +242,157 @@ def time_steps(launches, steps, stream, watch=None):
     return elapsed, {k: [a.elapsed_time(b) for a, b in v] for k, v in ev.items()}
 
 
+def host_cpu():
+    """Host CPU description from lscpu (logical CPUs, cores, model)."""
+    import subprocess
+    info = {"logical_cpus": os.cpu_count()}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        kv = {}
+        for line in txt.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        info["model"] = kv.get("Model name")
+        if "CPU(s)" in kv:
+            info["logical_cpus"] = int(kv["CPU(s)"])
+        if "Core(s) per socket" in kv and "Socket(s)" in kv:
+            info["physical_cores"] = int(kv["Core(s) per socket"]) * int(kv["Socket(s)"])
+        info["threads_per_core"] = int(kv.get("Thread(s) per core", "1"))
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    return info
+
+
 def cpu_baseline(wl, target_s):
     """The reference's CPU basket path on the host: its SOA3D basket
     virtuals, built from the reference's sources with its Release flags
-    (oracle/_ref/libvgref_fast.so), over contiguous ranges on every host core;
+    (oracle/_ref/libvgref_fast.so), over contiguous ranges of each basket;
     the oracle restatement stands in where that build is absent.  The same
-    queries on a bounded, evenly strided sample of the same inputs."""
+    queries on a bounded, evenly strided sample of the same inputs, timed on
+    1 thread and on every CPU this process may run on: one warm-up pass, then
+    the median of 5 passes (steady clock)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
     ref = po.reference(fast=True)
     o = ref if ref is not None and ref.has("basket") else po.oracle()
     kind = "reference" if o is ref else "port"
-    nthreads = max(1, min(16, os.cpu_count() or 1))
-    # Calibrate: one shape, all queries, small sample.
-    frac = 1.0 / 64
-    total_q = 0
-    reps = 1
-    t0 = time.perf_counter()
-    while True:
-        total_q = 0
+    host = host_cpu()
+    nall = max(1, host.get("affinity_cpus") or host.get("logical_cpus") or 1)
+    handles = [o.placed(u.type, u.params, xf.array()) for (name, u), xf in zip(wl.shapes, wl.xfs)]
+
+    def one_pass(stride, nthreads):
+        total = 0
         t0 = time.perf_counter()
-        for it, (name, u), xf in [z for _ in range(reps) for z in zip(wl.items, wl.shapes, wl.xfs)]:
-            h = o.placed(u.type, u.params, xf.array())
+        for it, h in zip(wl.items, handles):
             run = h.basket if kind == "reference" else h.query
             hp = it["host"]
-            step = int(round(1 / frac))
-            sel_all = slice(0, wl.n, step)
-            iout = it["outside_idx"][::step]
-            iin = it["inside_idx"][::step]
+            sel_all = slice(0, wl.n, stride)
+            iout = it["outside_idx"][::stride]
+            iin = it["inside_idx"][::stride]
             P, D, LP, LD = hp["p"], hp["d"], hp["lp"], hp["ld"]
             run(po.Q_CONTAINS, P[:, sel_all], nthreads=nthreads)
             run(po.Q_INSIDE, P[:, sel_all], nthreads=nthreads)
-            run(po.Q_DIST_IN, P[:, iout], D[:, iout], np.full(len(iout), 1e30), nthreads=nthreads)
+            if (it["name"], "dist_in") not in EXCLUDED:
+                run(po.Q_DIST_IN, P[:, iout], D[:, iout], np.full(len(iout), 1e30), nthreads=nthreads)
+                total += len(iout)
             run(po.Q_SAFE_IN, P[:, iout], nthreads=nthreads)
             run(po.Q_DIST_OUT, LP[:, iin], LD[:, iin], np.full(len(iin), 1e30), nthreads=nthreads)
             run(po.Q_SAFE_OUT, LP[:, iin], nthreads=nthreads)
-            total_q += 2 * len(range(0, wl.n, step)) + 2 * len(iout) + 2 * len(iin)
-        dt = time.perf_counter() - t0
-        if dt >= 0.5 * target_s:
+            total += 2 * len(range(0, wl.n, stride)) + len(iout) + 2 * len(iin)
+        return total, time.perf_counter() - t0
+
+    # Size the sample so that one single-thread pass takes about target_s / 7
+    # (6 single-thread passes and 6 shorter multi-thread ones fit target_s).
+    stride = 1024
+    want = target_s / 7
+    while True:
+        _, dt = one_pass(stride, 1)
+        if dt >= 0.6 * want or stride == 1:
             break
-        if frac >= 1.0:
-            reps = max(reps + 1, int(reps * target_s / max(dt, 1e-3)))
-        else:
-            frac = min(1.0, frac * max(2.0, target_s / max(dt, 1e-3)))
+        nxt = int(stride * dt / want) if dt > 0.05 else stride // 16
+        stride = max(1, min(stride - 1, nxt))
+    res = {}
+    for nt in (1, nall):
+        one_pass(stride, nt)  # warm-up
+        runs = [one_pass(stride, nt) for _ in range(5)]
+        q = runs[0][0]
+        med = float(np.median([dt for _, dt in runs]))
+        res[nt] = (q, med)
+    q, med_all = res[nall]
+    _, med_1 = res[1]
+    what = ("reference SOA3D basket virtuals, Scalar backend, Release flags (-O3 -ffast-math)" if kind == "reference"
+            else "oracle C++ restatement of the reference Scalar backend")
     return {
-        "value": total_q / dt / 1e9,
+        "value": round(q / med_all / 1e9, 5),
         "unit": "Gqueries/s",
-        "cores": nthreads,
+        "cores": nall,
         "kind": kind,
-        "sample": f"every {int(round(1 / frac))}th query of the same workload x{reps} ({total_q} queries, {dt:.1f} s), "
-                  + ("reference SOA3D basket virtuals, Scalar backend, Release flags (-O3 -ffast-math)"
-                     if kind == "reference" else "oracle C++ restatement of the reference Scalar backend")
-                  + ", std::thread over contiguous ranges",
+        "single_thread": {"value": round(q / med_1 / 1e9, 6), "unit": "Gqueries/s", "cores": 1,
+                          "median_s": round(med_1, 4)},
+        "median_s": round(med_all, 4),
+        "host_cpu": host,
+        "sample": f"every {stride}th query of the same workload ({q} queries per pass), {what}, std::thread over "
+                  f"contiguous ranges; one warm-up pass, median of 5 passes, on 1 thread and on all {nall} CPUs",
     }
+
+
+def sample_check(wl, promote, stride=61):
+    """Compares the step's current outputs with the oracle on every
+    stride-th query of each (shape, query), the oracle evaluated on the
+    inputs the device saw (float32 values promoted to double when promote).
+    Returns per shape: classification mismatches, distance/safety lanes
+    beyond max(1e-9 mm, 1e-12 relative), the largest |difference| and the
+    lanes compared."""
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    o = po.oracle()
+    cv = (lambda a: a.astype(np.float32).astype(np.float64)) if promote else (lambda a: a)
+    qcode = {"contains": po.Q_CONTAINS, "inside": po.Q_INSIDE, "dist_in": po.Q_DIST_IN, "safe_in": po.Q_SAFE_IN,
+             "dist_out": po.Q_DIST_OUT, "safe_out": po.Q_SAFE_OUT}
+    torch.cuda.synchronize()
+    out = {}
+    for it, (name, u), xf in zip(wl.items, wl.shapes, wl.xfs):
+        h = o.placed(u.type, u.params, xf.array())
+        hp = it["host"]
+        r = {"class_mismatch": 0, "beyond_bar": 0, "max_abs_err": 0.0, "lanes": 0}
+        for q in it["count"]:
+            dev = it["out"][q].cpu().numpy()
+            if q in ("contains", "inside"):
+                idx = np.arange(0, wl.n, stride)
+                want = h.query(qcode[q], cv(hp["p"][:, idx]))
+                r["class_mismatch"] += int((dev[idx].astype(np.int64) != want.astype(np.int64)).sum())
+            else:
+                src_idx = it["outside_idx"] if q in ("dist_in", "safe_in") else it["inside_idx"]
+                k = np.arange(0, len(src_idx), stride)
+                sel = src_idx[k]
+                P = cv(hp["p"][:, sel]) if q in ("dist_in", "safe_in") else cv(hp["lp"][:, sel])
+                if q in ("dist_in", "dist_out"):
+                    D = cv(hp["d"][:, sel]) if q == "dist_in" else cv(hp["ld"][:, sel])
+                    want = h.query(qcode[q], P, D, np.full(len(sel), 1e30))
+                else:
+                    want = h.query(qcode[q], P)
+                got = dev[k]
+                eq = (got == want) | (np.isnan(got) & np.isnan(want))
+                with np.errstate(invalid="ignore"):
+                    diff = np.where(eq, 0.0, np.abs(got - want))
+                    ok = eq | (diff <= np.maximum(1e-9, 1e-12 * np.abs(want)))
+                r["beyond_bar"] += int((~ok).sum())
+                fin = np.isfinite(diff)
+                if fin.any():
+                    r["max_abs_err"] = max(r["max_abs_err"], float(diff[fin].max()))
+                if (~fin & ~eq).any():
+                    r["max_abs_err"] = float("inf")
+            r["lanes"] += len(dev[::stride]) if q in ("contains", "inside") else len(np.arange(0, len(dev), stride))
+        out[name] = r
+    return out
 
 
 def batch_forms(args, wl, stream):
@@ -274,10 +407,29 @@ def batch_forms(args, wl, stream):
     bt = B.Batch(wl.geom, wl.tasks())
     best_ms = bt.autotune(3)
     it, blocks, items = bt.launch()
+    # Timed region: HIP events around each class kernel on the stream it runs
+    # on (vg_batch_set_timing); a handful of events per step.
+    bt.set_timing(True)
     el, _ = time_steps([(("batch", "all"), bt.run)], args.steps, stream, watch=[])
+    classes = [c for c in bt.class_stats() if c["runs"] > 0]
+    bt.set_timing(False)
     res = {"value": round(wl.queries_per_step * args.steps / el / 1e9, 3), "ms_per_step": round(1e3 * el / args.steps, 4),
            "items_per_lane": it, "blocks": blocks, "autotune_ms": round(best_ms, 4),
            "achieved_gbs": round(wl.bytes_per_step / (el / args.steps) / 1e9, 1)}
+    for c in classes:
+        c["achieved_gbs"] = round(c["bytes"] / (c["mean_ms"] * 1e-3) / 1e9, 1) if c["mean_ms"] > 0 else None
+        c["mean_us"] = round(1e3 * c.pop("mean_ms"), 3)
+    res["classes"] = classes
+    dom = max(classes, key=lambda c: c["mean_us"])
+    res["roofline"] = {"bound": "hbm", "achieved": dom["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(dom["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                       "traffic": profiled_class_traffic(dom["class"], dom["items"]),
+                       "kernel": f"BatchKernel<{ClassIT(dom['class'], it)},{dom['class']}> ({dom['name']})",
+                       "avg_launch_us": dom["mean_us"], "queries_per_launch": dom["items"],
+                       "bytes_per_launch": dom["bytes"],
+                       "bytes_rule": "each input element read once, each output element written once (FP64: "
+                                     "Contains 25 B, Inside 28, distances 64, safeties 32 per query)"}
+    res["check_vs_oracle"] = sample_check(wl, promote=False)
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
@@ -302,8 +454,9 @@ def batch_forms(args, wl, stream):
     fp32 = {"value": round(wl.queries_per_step * args.steps / el32 / 1e9, 3), "unit": "Gqueries/s",
             "ms_per_step": round(1e3 * el32 / args.steps, 4), "items_per_lane": it32, "blocks": blocks32,
             "achieved_gbs": round(wl.bytes_per_step_f32 / (el32 / args.steps) / 1e9, 1),
-            "note": "float32 inputs promoted to double before any arithmetic: results equal the FP64 form on the "
-                    "same float values (max error 0 vs the oracle on promoted inputs, classifications identical)"}
+            "note": "float32 inputs promoted to double before any arithmetic; check_vs_oracle compares a sample "
+                    "of the outputs with the oracle evaluated on the same promoted inputs"}
+    fp32["check_vs_oracle"] = sample_check(wl, promote=True)
     f32.free()
     return res, fp32
 
@@ -438,29 +591,39 @@ def serve_bench(args, dev):
 
 def nav_cpu_baseline(flat, p0, d0, target_s):
     """The reference's own ABBoxNavigator (built from its sources with its
-    Release flags) on one core, on a bounded prefix of the same tracks; the
-    navigator keeps a process-wide hit list, so it runs on one thread."""
+    Release flags) on one core, on a bounded prefix of the same tracks: one
+    warm-up pass, then the median of 5.  The navigator keeps a process-wide
+    hit list, so it runs on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
     ref = po.reference(fast=True)
     kind, lib, cores = "reference", ref, 1
     if ref is None:
-        kind, lib, cores = "port", po.oracle(), max(1, min(16, os.cpu_count() or 1))
+        kind, lib, cores = "port", po.oracle(), 1
     nav = po.Nav(lib, flat)
     m = 256
+    want = target_s / 6
     while True:
         t0 = time.perf_counter()
         ns, _, _, _ = nav.trace(p0[:, :m], d0[:, :m], kind=0, nthreads=cores)
         dt = time.perf_counter() - t0
-        if dt >= 0.5 * target_s or m >= p0.shape[1]:
+        if dt >= 0.6 * want or m >= p0.shape[1]:
             break
-        m = min(p0.shape[1], int(m * max(2.0, target_s / max(dt, 1e-3))))
+        m = min(p0.shape[1], max(m + 1, int(m * want / max(dt, 1e-3))))
+    runs = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        ns, _, _, _ = nav.trace(p0[:, :m], d0[:, :m], kind=0, nthreads=cores)
+        runs.append(time.perf_counter() - t0)
+    dt = float(np.median(runs))
     steps = int(ns.sum())
     return {"value": round(steps / dt / 1e9, 6), "unit": "G steps/s", "cores": cores, "kind": kind,
-            "sample": f"first {m} tracks of the same workload ({steps} steps, {dt:.1f} s), "
+            "median_s": round(dt, 4),
+            "sample": f"first {m} tracks of the same workload ({steps} steps per pass), "
                       + ("reference ABBoxNavigator, Release flags (-O3 -ffast-math)" if kind == "reference"
-                         else "oracle C++ restatement")}
+                         else "oracle C++ restatement")
+                      + "; one warm-up pass, median of 5 passes on one thread"}
 
 
 def main():
@@ -563,6 +726,11 @@ def main():
     best_form = max(forms, key=forms.get)
     value_per_launch = value
     value = forms[best_form]
+    roof_per_launch = roof
+    if best_form.startswith("batched"):
+        # The roofline of the form that produced value: its dominant class
+        # kernel, event-timed inside the batched timed region.
+        roof = batched["roofline"]
     ms_per_step = 1e3 * wl.queries_per_step / (value * 1e9)
 
     cpu = None
@@ -584,6 +752,7 @@ def main():
                    "points_per_shape": args.points, "shapes": [nm for nm, _ in wl.shapes],
                    "shapes_without_kernels": wl.missing, "queries_per_step": wl.queries_per_step},
         "roofline": roof,
+        "roofline_per_launch_form": roof_per_launch,
         "forms_gqps": {k: round(v, 3) for k, v in forms.items()},
         "per_launch_value": round(value_per_launch, 3),
         "batched": batched,

@@ -266,13 +266,28 @@ typedef struct vg_batch vg_batch;
 int vg_batch_create(const vg_geometry *g, const vg_query_task *tasks, int32_t ntasks, vg_batch **out);
 int vg_batch_run(vg_batch *b, void *stream);
 /* Launch shape: items per lane (1, 2, 4) and the number of persistent
- * workgroups (0 = one workgroup per tile).  Synchronises the device. */
+ * workgroups (0 = one workgroup per tile).  Each items-per-lane value has
+ * its own device task table, written once by vg_batch_create, so changing
+ * the launch shape rewrites no device memory: launches in flight and
+ * hipGraphs captured earlier keep their own shape (capture again to replay
+ * the new one). */
 int vg_batch_set_launch(vg_batch *b, int32_t items_per_lane, uint32_t blocks);
 int vg_batch_get_launch(const vg_batch *b, int32_t *items_per_lane, uint32_t *blocks, uint64_t *items);
 /* Times every launch shape on the batch itself (`repetitions` runs each) and
- * keeps the fastest; *best_ms (nullable) receives its time per run.  Runs the
- * batch, so outputs are written; synchronises the device. */
+ * keeps the fastest, as vg_batch_set_launch would; *best_ms (nullable)
+ * receives its time per run.  Runs the batch, so outputs are written; waits
+ * for its own timing events. */
 int vg_batch_autotune(vg_batch *b, int32_t repetitions, void *stream, float *best_ms);
+/* Per-class timing of later vg_batch_run calls: when enabled, HIP events are
+ * recorded around each class kernel on the stream it is launched on (the
+ * last 64 runs are kept).  Enabling clears earlier records. */
+int vg_batch_set_timing(vg_batch *b, int32_t enable);
+/* Weight class `cls` (0 light, 1 Cone/Sphere, 2 Torus, 3 Polycone/Polyhedron
+ * staged in LDS, 4 the same read from global memory): its queries per run,
+ * its algorithmic bytes per run (each input element read once, each output
+ * written once), and the number of timed runs and their mean kernel
+ * duration (waits for the recorded events).  Each pointer is nullable. */
+int vg_batch_class_stats(vg_batch *b, int32_t cls, uint64_t *items, uint64_t *bytes, int32_t *runs, float *mean_ms);
 int vg_batch_free(vg_batch *b);
 
 /* -------------------------------------------------------------- navigation */
@@ -398,10 +413,14 @@ int vg_server_submit(vg_server *s, int64_t basket, int64_t *request);
 int vg_server_wait(vg_server *s, int64_t request, double *latency_s);
 /* Non-blocking: *done = 1 (and latency_s set) once complete. */
 int vg_server_poll(vg_server *s, int64_t request, int32_t *done, double *latency_s);
+/* Number of tracks of an open basket. */
+int vg_server_basket_size(vg_server *s, int64_t basket, uint64_t *n);
 /* Copies the basket's current track states to host arrays (each nullable;
- * synchronous).  Fails with VG_ERR_STATE while a step of the basket runs. */
-int vg_server_read(vg_server *s, int64_t basket, double *x, double *y, double *z, int32_t *level, int32_t *boundary,
-                   int32_t *nsteps, uint64_t *hash, double *last_step);
+ * synchronous).  `capacity` is the number of entries every non-null array
+ * holds: VG_ERR_CAPACITY (nothing copied) if the basket has more tracks.
+ * Fails with VG_ERR_STATE while a step of the basket runs. */
+int vg_server_read(vg_server *s, int64_t basket, uint64_t capacity, double *x, double *y, double *z, int32_t *level,
+                   int32_t *boundary, int32_t *nsteps, uint64_t *hash, double *last_step);
 /* Frees the basket's slots; VG_ERR_STATE while it has queued or running steps. */
 int vg_server_close(vg_server *s, int64_t basket);
 /* Launches made, requests they carried, and track steps taken. */

@@ -45,7 +45,14 @@ capi.register({
                             ctypes.POINTER(ctypes.c_uint64)],
     "vg_batch_autotune": [c_vp, ctypes.c_int32, c_vp, ctypes.POINTER(ctypes.c_float)],
     "vg_batch_free": [c_vp],
+    "vg_batch_set_timing": [c_vp, ctypes.c_int32],
+    "vg_batch_class_stats": [c_vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                             ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_float)],
 })
+
+# Weight classes of a batch (one kernel launch each).
+CLASS_NAMES = ["light (Box, Tube, Orb, Trd, Trapezoid, Parallelepiped, Paraboloid)", "Cone, Sphere", "Torus",
+               "Polycone, Polyhedron (parameters staged in LDS)", "Polycone, Polyhedron (parameters in global memory)"]
 
 
 def _stream():
@@ -59,17 +66,25 @@ class Task:
     def __init__(self, placed, query, p, d=None, step_max=None, out=None):
         if len(p) != 3 or (query in DIRECTIONAL and (d is None or len(d) != 3)):
             raise ValueError("points (and directions for distance queries) must be (x, y, z) triples")
+        if query not in range(7):
+            raise ValueError(f"unknown query code {query}")
         dt = p[0].dtype
         if dt not in (torch.float64, torch.float32):
             raise ValueError("inputs must be float64 or float32")
         n = p[0].numel()
+        dev = p[0].device
         for a in list(p) + (list(d) if d is not None else []):
-            if a.dtype != dt or a.numel() != n or not a.is_cuda or not a.is_contiguous():
-                raise ValueError("inputs must be contiguous CUDA tensors of one dtype and length")
-        if step_max is not None and (step_max.dtype != torch.float64 or step_max.numel() != n):
-            raise ValueError("step_max must be float64 with one entry per point")
+            if a.dtype != dt or a.numel() != n or not a.is_cuda or a.device != dev or not a.is_contiguous():
+                raise ValueError("inputs must be contiguous CUDA tensors of one dtype, length and device")
+        if step_max is not None and (step_max.dtype != torch.float64 or step_max.numel() != n or not step_max.is_cuda
+                                     or step_max.device != dev or not step_max.is_contiguous()):
+            raise ValueError("step_max must be a contiguous float64 CUDA tensor with one entry per point")
+        odt = OUT_DTYPE.get(query, torch.float64)
         if out is None:
-            out = torch.empty(n, dtype=OUT_DTYPE.get(query, torch.float64), device=p[0].device)
+            out = torch.empty(n, dtype=odt, device=dev)
+        elif (out.dtype != odt or out.numel() != n or not out.is_cuda or out.device != dev
+              or not out.is_contiguous()):
+            raise ValueError(f"out must be a contiguous {odt} CUDA tensor with one entry per point")
         self.placed, self.query, self.p, self.d, self.step_max, self.out, self.n = placed, query, p, d, step_max, out, n
         self.input_type = F32 if dt == torch.float32 else F64
 
@@ -113,6 +128,21 @@ class Batch:
         ms = ctypes.c_float()
         capi.call("vg_batch_autotune", self.handle, int(repetitions), _stream(), ctypes.byref(ms))
         return ms.value
+
+    def set_timing(self, enable=True):
+        capi.call("vg_batch_set_timing", self.handle, 1 if enable else 0)
+
+    def class_stats(self):
+        """Per weight class: queries and algorithmic bytes per run, timed runs
+        and mean kernel duration (ms) since timing was enabled."""
+        out = []
+        for c in range(len(CLASS_NAMES)):
+            it, by, runs, ms = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int32(), ctypes.c_float()
+            capi.call("vg_batch_class_stats", self.handle, c, ctypes.byref(it), ctypes.byref(by), ctypes.byref(runs),
+                      ctypes.byref(ms))
+            out.append({"class": c, "name": CLASS_NAMES[c], "items": it.value, "bytes": by.value, "runs": runs.value,
+                        "mean_ms": ms.value})
+        return out
 
     def free(self):
         if self.handle:

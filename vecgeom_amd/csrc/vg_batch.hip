@@ -387,7 +387,12 @@ struct vg_batch {
   int first[vga::kClasses + 1] = {0, 0, 0, 0, 0, 0};  // class c owns host[first[c] .. first[c+1])
   uint64_t ntiles[vga::kClasses] = {0, 0, 0, 0, 0};   // tiles per class for the current IT
   int stage_len = 0;                               // LDS doubles for the staged class
-  TaskDev *d_tasks = nullptr;
+  // One device task table per items-per-lane value (1, 2, 4), each written
+  // once at create: changing the launch shape only selects another table,
+  // so a hipGraph captured earlier keeps a table that still matches its
+  // own grid and kernel.
+  TaskDev *d_table[3] = {nullptr, nullptr, nullptr};
+  TaskDev *d_tasks = nullptr;  // the table of the current IT
   int it = 2;            // items per lane
   unsigned blocks = 0;   // 0 = one workgroup per tile, else a persistent grid per class
   uint64_t items = 0;
@@ -397,6 +402,16 @@ struct vg_batch {
   hipStream_t side[vga::kClasses - 1] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t fork = nullptr;
   hipEvent_t join[vga::kClasses - 1] = {nullptr, nullptr, nullptr, nullptr};
+  // Per-class timing (vg_batch_set_timing): a ring of event pairs recorded
+  // on the stream each class kernel is launched on, around that kernel.
+  static constexpr int kRing = 64;
+  bool timing = false;
+  int ring_used = 0, ring_next = 0;
+  hipEvent_t t0[vga::kClasses][kRing] = {};
+  hipEvent_t t1[vga::kClasses][kRing] = {};
+  bool recorded[vga::kClasses][kRing] = {};
+  uint64_t class_items[vga::kClasses] = {0, 0, 0, 0, 0};
+  uint64_t class_bytes[vga::kClasses] = {0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -461,7 +476,23 @@ void LaunchIT(const vg_batch *b, unsigned grid, hipStream_t s) {
     LaunchClass<4, CLS>(b, grid, s);
 }
 
+// Algorithmic bytes of one task: every input element read once, every
+// output element written once.
+uint64_t TaskBytes(const TaskDev &t) {
+  const uint64_t w = t.in_type == VG_F32 ? 4 : 8;
+  const bool dir = t.query == kQDistIn || t.query == kQDistOut || t.query == kQPlacedDistOut;
+  uint64_t per = 3 * w;
+  if (dir) per += 3 * w + (t.sm ? 8 : 0);
+  per += t.query == kQContains ? 1 : (t.query == kQInside ? 4 : 8);
+  return per * t.n;
+}
+
 int Launch(vg_batch *b, hipStream_t s) {
+  const int slot = b->ring_next;
+  if (b->timing) {
+    b->ring_next = (b->ring_next + 1) % vg_batch::kRing;
+    if (b->ring_used < vg_batch::kRing) ++b->ring_used;
+  }
   int classes = 0;
   for (int c = 0; c < kClasses; ++c) classes += b->ntiles[c] ? 1 : 0;
   int rc = VG_OK;
@@ -481,6 +512,10 @@ int Launch(vg_batch *b, hipStream_t s) {
     }
     first = false;
     const unsigned grid = (c == 0 && b->blocks) ? (unsigned)std::min<uint64_t>(b->blocks, nt) : (unsigned)nt;
+    if (b->timing) {
+      rc = CheckHip(hipEventRecord(b->t0[c][slot], cs), "batch timing");
+      if (rc) break;
+    }
     if (c == 0)
       LaunchIT<0>(b, grid, cs);
     else if (c == 1)
@@ -492,6 +527,10 @@ int Launch(vg_batch *b, hipStream_t s) {
     else
       LaunchIT<4>(b, grid, cs);
     rc = CheckHip(hipGetLastError(), "batch launch");
+    if (!rc && b->timing) {
+      rc = CheckHip(hipEventRecord(b->t1[c][slot], cs), "batch timing");
+      b->recorded[c][slot] = true;
+    }
     if (!rc && cs != s) {
       rc = CheckHip(hipEventRecord(b->join[nside], cs), "batch join");
       if (!rc) rc = CheckHip(hipStreamWaitEvent(s, b->join[nside], 0), "batch join wait");
@@ -501,7 +540,17 @@ int Launch(vg_batch *b, hipStream_t s) {
   return rc;
 }
 
+void DestroyTiming(vg_batch *b) {
+  for (int c = 0; c < kClasses; ++c)
+    for (int k = 0; k < vg_batch::kRing; ++k) {
+      if (b->t0[c][k]) (void)hipEventDestroy(b->t0[c][k]);
+      if (b->t1[c][k]) (void)hipEventDestroy(b->t1[c][k]);
+      b->t0[c][k] = b->t1[c][k] = nullptr;
+    }
+}
+
 void DestroySync(vg_batch *b) {
+  DestroyTiming(b);
   for (auto &x : b->side)
     if (x) (void)hipStreamDestroy(x);
   for (auto &x : b->join)
@@ -518,15 +567,38 @@ int CreateSync(vg_batch *b) {
   return rc;
 }
 
-// Rewrites the device task table for the current IT.  Launches already
-// enqueued may still read the old table, so the device is drained first.
-int Upload(vg_batch *b) {
+int ITIndex(int it) { return it == 1 ? 0 : (it == 2 ? 1 : 2); }
+
+// Writes the device task table of every IT (at create; synchronous), then
+// selects the current one.
+int UploadAll(vg_batch *b) {
+  const int keep = b->it;
+  int rc = VG_OK;
+  for (int k = 0; k < 3 && !rc; ++k) {
+    b->it = kITs[k];
+    Schedule(b);
+    rc = CheckHip(hipMalloc(&b->d_table[k], b->host.size() * sizeof(TaskDev)), "hipMalloc(batch table)");
+    if (!rc)
+      rc = CheckHip(hipMemcpy(b->d_table[k], b->host.data(), b->host.size() * sizeof(TaskDev), hipMemcpyHostToDevice),
+                    "batch table upload");
+  }
+  b->it = keep;
   Schedule(b);
-  if (b->host.empty()) return VG_OK;
-  int rc = CheckHip(hipDeviceSynchronize(), "batch table: device synchronize");
-  if (rc) return rc;
-  return CheckHip(hipMemcpy(b->d_tasks, b->host.data(), b->host.size() * sizeof(TaskDev), hipMemcpyHostToDevice),
-                  "batch table upload");
+  b->d_tasks = b->d_table[ITIndex(b->it)];
+  return rc;
+}
+
+// Selects the tables and tile counts of the current IT (host only: no
+// device memory is rewritten, so launches in flight and captured graphs are
+// unaffected).
+void Select(vg_batch *b) {
+  Schedule(b);
+  b->d_tasks = b->d_table[ITIndex(b->it)];
+}
+
+void FreeTables(vg_batch *b) {
+  for (auto &t : b->d_table)
+    if (t) (void)hipFree(t);
 }
 
 // Validates one task and fills its device record; returns a status.
@@ -608,13 +680,15 @@ int vg_batch_create(const vg_geometry *g, const vg_query_task *tasks, int32_t nt
   }
   for (int k = b->first[kStagedClass]; k < b->first[kStagedClass + 1]; ++k)
     b->stage_len = std::max(b->stage_len, b->host[k].param_len);
+  for (int c = 0; c < kClasses; ++c)
+    for (int k = b->first[c]; k < b->first[c + 1]; ++k) {
+      b->class_items[c] += b->host[k].n;
+      b->class_bytes[c] += TaskBytes(b->host[k]);
+    }
   rc = CreateSync(b);
-  if (!rc && !b->host.empty()) {
-    rc = CheckHip(hipMalloc(&b->d_tasks, b->host.size() * sizeof(TaskDev)), "hipMalloc(batch table)");
-    if (!rc) rc = Upload(b);
-  }
+  if (!rc && !b->host.empty()) rc = UploadAll(b);
   if (rc) {
-    if (b->d_tasks) (void)hipFree(b->d_tasks);
+    FreeTables(b);
     DestroySync(b);
     delete b;
     return rc;
@@ -663,7 +737,8 @@ int vg_batch_set_launch(vg_batch *b, int32_t items_per_lane, uint32_t blocks) {
   if (rc) return rc;
   b->it = items_per_lane;
   b->blocks = blocks;
-  return Upload(b);
+  Select(b);
+  return VG_OK;
 }
 
 int vg_batch_get_launch(const vg_batch *b, int32_t *items_per_lane, uint32_t *blocks, uint64_t *items) {
@@ -698,7 +773,7 @@ int vg_batch_autotune(vg_batch *b, int32_t repetitions, void *stream, float *bes
     for (unsigned gr : grids) {
       b->it = it;
       b->blocks = gr;
-      if ((rc = Upload(b))) break;
+      Select(b);
       if ((rc = Launch(b, s))) break;  // warm-up
       (void)hipEventRecord(e0, s);
       for (int r = 0; r < repetitions && !rc; ++r) rc = Launch(b, s);
@@ -719,14 +794,58 @@ int vg_batch_autotune(vg_batch *b, int32_t repetitions, void *stream, float *bes
   (void)hipEventDestroy(e1);
   b->it = bestIt;
   b->blocks = bestBlocks;
-  int rc2 = Upload(b);
+  Select(b);
   if (best_ms) *best_ms = best;
-  return rc ? rc : rc2;
+  return rc;
+}
+
+int vg_batch_set_timing(vg_batch *b, int32_t enable) {
+  if (!b) return SetError(VG_ERR_INVALID, "null batch");
+  int rc = EnsureDevice();
+  if (rc) return rc;
+  if (enable && !b->t0[0][0]) {
+    for (int c = 0; c < kClasses && !rc; ++c)
+      for (int k = 0; k < vg_batch::kRing && !rc; ++k) {
+        rc = CheckHip(hipEventCreate(&b->t0[c][k]), "hipEventCreate");
+        if (!rc) rc = CheckHip(hipEventCreate(&b->t1[c][k]), "hipEventCreate");
+      }
+    if (rc) {
+      DestroyTiming(b);
+      return rc;
+    }
+  }
+  b->timing = enable != 0;
+  b->ring_used = b->ring_next = 0;
+  std::memset(b->recorded, 0, sizeof(b->recorded));
+  return VG_OK;
+}
+
+int vg_batch_class_stats(vg_batch *b, int32_t cls, uint64_t *items, uint64_t *bytes, int32_t *runs, float *mean_ms) {
+  if (!b || cls < 0 || cls >= kClasses) return SetError(VG_ERR_INVALID, "null batch or class out of range");
+  if (items) *items = b->class_items[cls];
+  if (bytes) *bytes = b->class_bytes[cls];
+  int n = 0;
+  double sum = 0;
+  if (mean_ms || runs) {
+    int rc = EnsureDevice();
+    if (rc) return rc;
+    for (int k = 0; k < b->ring_used; ++k) {
+      if (!b->recorded[cls][k]) continue;
+      if ((rc = CheckHip(hipEventSynchronize(b->t1[cls][k]), "batch timing sync"))) return rc;
+      float ms = 0;
+      if ((rc = CheckHip(hipEventElapsedTime(&ms, b->t0[cls][k], b->t1[cls][k]), "batch timing read"))) return rc;
+      sum += ms;
+      ++n;
+    }
+  }
+  if (runs) *runs = n;
+  if (mean_ms) *mean_ms = n ? (float)(sum / n) : 0.f;
+  return VG_OK;
 }
 
 int vg_batch_free(vg_batch *b) {
   if (!b) return SetError(VG_ERR_INVALID, "null batch");
-  if (b->d_tasks) (void)hipFree(b->d_tasks);
+  FreeTables(b);
   DestroySync(b);
   delete b;
   return VG_OK;

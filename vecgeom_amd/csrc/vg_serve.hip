@@ -534,8 +534,17 @@ int vg_server_poll(vg_server *s, int64_t request, int32_t *done, double *latency
   return VG_OK;
 }
 
-int vg_server_read(vg_server *s, int64_t basket, double *x, double *y, double *z, int32_t *level, int32_t *boundary,
-                   int32_t *nsteps, uint64_t *hash, double *last_step) {
+int vg_server_basket_size(vg_server *s, int64_t basket, uint64_t *n) {
+  if (!s || !n) return SetError(VG_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->baskets.find(basket);
+  if (it == s->baskets.end()) return SetError(VG_ERR_INVALID, "unknown basket");
+  *n = it->second.n;
+  return VG_OK;
+}
+
+int vg_server_read(vg_server *s, int64_t basket, uint64_t capacity, double *x, double *y, double *z, int32_t *level,
+                   int32_t *boundary, int32_t *nsteps, uint64_t *hash, double *last_step) {
   if (!s) return SetError(VG_ERR_INVALID, "null server");
   uint64_t off, n;
   {
@@ -546,6 +555,9 @@ int vg_server_read(vg_server *s, int64_t basket, double *x, double *y, double *z
     off = it->second.offset;
     n = it->second.n;
   }
+  if (n > capacity)
+    return SetError(VG_ERR_CAPACITY, "basket holds " + std::to_string(n) + " tracks, the arrays " +
+                                         std::to_string(capacity));
   int rc = EnsureDevice();
   struct Out {
     void *dst;

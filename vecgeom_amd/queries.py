@@ -14,14 +14,38 @@ def _stream():
     return torch.cuda.current_stream().cuda_stream
 
 
-def _check_soa(p, name):
+def _check_soa(p, name, n=None, device=None):
+    """An (x, y, z) triple of contiguous float64 CUDA tensors of one length
+    (n, when given) on one device (device, when given); returns the length."""
     if len(p) != 3:
         raise ValueError(f"{name} must be an (x, y, z) triple")
-    n = p[0].numel()
+    n = p[0].numel() if n is None else n
+    device = p[0].device if device is None else device
     for a in p:
-        if a.dtype != torch.float64 or not a.is_cuda or a.numel() != n or not a.is_contiguous():
-            raise ValueError(f"{name} components must be contiguous float64 CUDA tensors of equal length")
+        if (not isinstance(a, torch.Tensor) or a.dtype != torch.float64 or not a.is_cuda or a.device != device
+                or a.numel() != n or not a.is_contiguous()):
+            raise ValueError(f"{name} components must be contiguous float64 CUDA tensors of length {n} on {device}")
     return n
+
+
+def _check_arr(a, name, dtype, n, device):
+    """One caller array the kernel reads or writes: a contiguous CUDA tensor
+    of the given dtype with n entries on the points' device.  A host tensor
+    or a shorter one would hand the kernel a pointer it must not touch."""
+    if (not isinstance(a, torch.Tensor) or a.dtype != dtype or not a.is_cuda or a.device != device
+            or a.numel() != n or not a.is_contiguous()):
+        raise ValueError(f"{name} must be a contiguous {dtype} CUDA tensor of length {n} on {device}")
+    return a
+
+
+def _out(out, dtype, n, device):
+    if out is None:
+        return torch.empty(n, dtype=dtype, device=device)
+    return _check_arr(out, "out", dtype, n, device)
+
+
+def _step_max(step_max, n, device):
+    return None if step_max is None else _check_arr(step_max, "step_max", torch.float64, n, device).data_ptr()
 
 
 def _ptrs(p):
@@ -30,40 +54,42 @@ def _ptrs(p):
 
 def contains(geom, placed, p, out=None):
     n = _check_soa(p, "points")
-    out = torch.empty(n, dtype=torch.uint8, device=p[0].device) if out is None else out
+    out = _out(out, torch.uint8, n, p[0].device)
     capi.call("vg_contains", geom.handle, int(placed), *_ptrs(p), n, out.data_ptr(), _stream())
     return out
 
 
 def inside(geom, placed, p, out=None):
     n = _check_soa(p, "points")
-    out = torch.empty(n, dtype=torch.int32, device=p[0].device) if out is None else out
+    out = _out(out, torch.int32, n, p[0].device)
     capi.call("vg_inside", geom.handle, int(placed), *_ptrs(p), n, out.data_ptr(), _stream())
     return out
 
 
 def distance_to_in(geom, placed, p, d, step_max=None, out=None):
     n = _check_soa(p, "points")
-    _check_soa(d, "directions")
-    out = torch.empty(n, dtype=torch.float64, device=p[0].device) if out is None else out
-    sm = step_max.data_ptr() if step_max is not None else None
+    _check_soa(d, "directions", n, p[0].device)
+    out = _out(out, torch.float64, n, p[0].device)
+    sm = _step_max(step_max, n, p[0].device)
     capi.call("vg_distance_to_in", geom.handle, int(placed), *_ptrs(p), *_ptrs(d), sm, n, out.data_ptr(), _stream())
     return out
 
 
 def distance_to_in_minimize(geom, placed, daughter_index, p, d, inout_dist, inout_next):
     n = _check_soa(p, "points")
-    _check_soa(d, "directions")
+    _check_soa(d, "directions", n, p[0].device)
+    _check_arr(inout_dist, "inout_dist", torch.float64, n, p[0].device)
+    _check_arr(inout_next, "inout_next", torch.int32, n, p[0].device)
     capi.call("vg_distance_to_in_minimize", geom.handle, int(placed), int(daughter_index), *_ptrs(p), *_ptrs(d), n,
               inout_dist.data_ptr(), inout_next.data_ptr(), _stream())
 
 
 def distance_to_out(geom, placed, p, d, step_max=None, out=None, next_node=None):
     n = _check_soa(p, "points")
-    _check_soa(d, "directions")
-    out = torch.empty(n, dtype=torch.float64, device=p[0].device) if out is None else out
-    sm = step_max.data_ptr() if step_max is not None else None
-    nx = next_node.data_ptr() if next_node is not None else None
+    _check_soa(d, "directions", n, p[0].device)
+    out = _out(out, torch.float64, n, p[0].device)
+    sm = _step_max(step_max, n, p[0].device)
+    nx = None if next_node is None else _check_arr(next_node, "next_node", torch.int32, n, p[0].device).data_ptr()
     capi.call("vg_distance_to_out", geom.handle, int(placed), *_ptrs(p), *_ptrs(d), sm, n, out.data_ptr(), nx,
               _stream())
     return out
@@ -71,9 +97,9 @@ def distance_to_out(geom, placed, p, d, step_max=None, out=None, next_node=None)
 
 def placed_distance_to_out(geom, placed, p, d, step_max=None, out=None):
     n = _check_soa(p, "points")
-    _check_soa(d, "directions")
-    out = torch.empty(n, dtype=torch.float64, device=p[0].device) if out is None else out
-    sm = step_max.data_ptr() if step_max is not None else None
+    _check_soa(d, "directions", n, p[0].device)
+    out = _out(out, torch.float64, n, p[0].device)
+    sm = _step_max(step_max, n, p[0].device)
     capi.call("vg_placed_distance_to_out", geom.handle, int(placed), *_ptrs(p), *_ptrs(d), sm, n, out.data_ptr(),
               _stream())
     return out
@@ -81,25 +107,27 @@ def placed_distance_to_out(geom, placed, p, d, step_max=None, out=None):
 
 def safety_to_in(geom, placed, p, out=None):
     n = _check_soa(p, "points")
-    out = torch.empty(n, dtype=torch.float64, device=p[0].device) if out is None else out
+    out = _out(out, torch.float64, n, p[0].device)
     capi.call("vg_safety_to_in", geom.handle, int(placed), *_ptrs(p), n, out.data_ptr(), _stream())
     return out
 
 
 def safety_to_in_minimize(geom, placed, p, inout):
     n = _check_soa(p, "points")
+    _check_arr(inout, "inout", torch.float64, n, p[0].device)
     capi.call("vg_safety_to_in_minimize", geom.handle, int(placed), *_ptrs(p), n, inout.data_ptr(), _stream())
 
 
 def safety_to_out(geom, placed, p, out=None):
     n = _check_soa(p, "points")
-    out = torch.empty(n, dtype=torch.float64, device=p[0].device) if out is None else out
+    out = _out(out, torch.float64, n, p[0].device)
     capi.call("vg_safety_to_out", geom.handle, int(placed), *_ptrs(p), n, out.data_ptr(), _stream())
     return out
 
 
 def safety_to_out_minimize(geom, placed, p, inout):
     n = _check_soa(p, "points")
+    _check_arr(inout, "inout", torch.float64, n, p[0].device)
     capi.call("vg_safety_to_out_minimize", geom.handle, int(placed), *_ptrs(p), n, inout.data_ptr(), _stream())
 
 

@@ -28,7 +28,8 @@ capi.register({
     "vg_server_submit": [c_vp, c_i64, P(c_i64)],
     "vg_server_wait": [c_vp, c_i64, P(c_dbl)],
     "vg_server_poll": [c_vp, c_i64, P(c_i32), P(c_dbl)],
-    "vg_server_read": [c_vp, c_i64] + [c_vp] * 8,
+    "vg_server_read": [c_vp, c_i64, c_u64] + [c_vp] * 8,
+    "vg_server_basket_size": [c_vp, c_i64, P(c_u64)],
     "vg_server_close": [c_vp, c_i64],
     "vg_server_stats": [c_vp, P(c_u64), P(c_u64), P(c_u64)],
     "vg_server_destroy": [c_vp],
@@ -60,19 +61,29 @@ class Server:
         capi.call("vg_server_capacity", self.handle, ctypes.byref(tot), ctypes.byref(free))
         return tot.value, free.value
 
+    @staticmethod
+    def _tracks(p, d):
+        """Six contiguous float64 host arrays of one length (the C call reads
+        n entries from each)."""
+        if len(p) != 3 or len(d) != 3:
+            raise ValueError("p and d must be (x, y, z) triples")
+        arrs = [_f64(a).ravel() for a in list(p) + list(d)]
+        if any(a.size != arrs[0].size for a in arrs):
+            raise ValueError("positions and directions must all hold the same number of tracks")
+        return arrs
+
     def open(self, p, d):
         """Admits a basket of tracks (host arrays, shape (3, n)); returns its id."""
-        p = [_f64(a) for a in p]
-        d = [_f64(a) for a in d]
-        n = p[0].size
+        arrs = self._tracks(p, d)
         b = c_i64()
-        capi.call("vg_server_open", self.handle, n, *[a.ctypes.data for a in p + d], ctypes.byref(b))
+        capi.call("vg_server_open", self.handle, arrs[0].size, *[a.ctypes.data for a in arrs], ctypes.byref(b))
         return b.value
 
     def try_open(self, p, d):
         """open(), but returns None when the basket does not fit the budget."""
-        p = [_f64(a) for a in p]
-        d = [_f64(a) for a in d]
+        arrs = self._tracks(p, d)
+        p = arrs[:3]
+        d = arrs[3:]
         b = c_i64()
         rc, msg = capi.status("vg_server_open", self.handle, p[0].size, *[a.ctypes.data for a in p + d],
                               ctypes.byref(b))
@@ -99,13 +110,23 @@ class Server:
         capi.call("vg_server_poll", self.handle, int(request), ctypes.byref(done), ctypes.byref(lat))
         return lat.value if done.value else None
 
-    def read(self, basket, n):
-        """The basket's n tracks: dict of position (3, n), level, boundary,
-        nsteps, hash, last_step."""
+    def basket_size(self, basket):
+        n = c_u64()
+        capi.call("vg_server_basket_size", self.handle, int(basket), ctypes.byref(n))
+        return n.value
+
+    def read(self, basket, n=None):
+        """The basket's tracks: dict of x, y, z, level, boundary, nsteps,
+        hash, last_step.  The arrays are sized by the basket itself; n, if
+        given, must equal its track count."""
+        size = self.basket_size(basket)
+        if n is not None and int(n) != size:
+            raise ValueError(f"basket {basket} holds {size} tracks, not {n}")
+        n = size
         out = {"x": np.empty(n), "y": np.empty(n), "z": np.empty(n), "level": np.empty(n, np.int32),
                "boundary": np.empty(n, np.int32), "nsteps": np.empty(n, np.int32),
                "hash": np.empty(n, np.uint64), "last_step": np.empty(n)}
-        capi.call("vg_server_read", self.handle, int(basket),
+        capi.call("vg_server_read", self.handle, int(basket), n,
                   *[out[k].ctypes.data for k in ("x", "y", "z", "level", "boundary", "nsteps", "hash", "last_step")])
         return out
 
