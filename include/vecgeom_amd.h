@@ -230,9 +230,17 @@ int vg_safety_to_out_minimize(const vg_geometry *g, int32_t placed, const double
 /* Input element types: VG_F64 = double arrays (the SOA3D<double> of the
  * reference); VG_F32 = float arrays (FP32 storage: every value is promoted to
  * double before any arithmetic, so results equal the FP64 form on the same
- * float-valued inputs).  Outputs are as for the FP64 entry points. */
+ * float-valued inputs); VG_MIXED = float arrays with FP32 arithmetic first:
+ * Contains and Inside are decided in FP32 wherever an FP32 lower bound on
+ * the point's distance to the surface exceeds its rigorous rounding margin,
+ * DistanceToIn returns kInfinity in FP32 for rays that provably miss the
+ * shape's extent box, and every other query (and every undecided lane) is
+ * evaluated by the FP64 kernel on the promoted inputs -- so every result
+ * equals the VG_F32 form's, bit for bit.  Outputs are as for the FP64 entry
+ * points. */
 #define VG_F64 0
 #define VG_F32 1
+#define VG_MIXED 2
 
 typedef struct vg_query_task {
   int32_t placed;     /* placed-volume id */
@@ -244,6 +252,10 @@ typedef struct vg_query_task {
   const void *dx, *dy, *dz; /* directions (distance queries only) */
   const double *step_max;   /* distance queries; NULL = kInfinity */
   void *out;                /* device array, see the query codes */
+  uint64_t *fp64_count;     /* nullable device counter: each run adds the
+                               number of queries it evaluated in FP64 (all of
+                               them for VG_F64 / VG_F32; the undecided ones
+                               for VG_MIXED).  Costs one atomic per wave. */
 } vg_query_task;
 
 /* One task on `stream`: the SOA3D basket virtuals of VPlacedVolume
@@ -314,9 +326,15 @@ typedef struct vg_nav_states {
 } vg_nav_states;
 
 /* Navigator selection (navigation/SimpleNavigator.h:341-482,
- * navigation/ABBoxNavigator.h:201-424). */
+ * navigation/ABBoxNavigator.h:201-424).  VG_NAV_ABBOX_MIXED is the ABBox
+ * navigator with an FP32 filter in its daughter walk: a daughter whose FP32
+ * lower bound on the distance from the point (with a rigorous rounding
+ * margin) exceeds the best step found so far is not evaluated in FP64.  Its
+ * steps, next volumes and states are those of VG_NAV_ABBOX, bit for bit; no
+ * counterpart in the reference. */
 #define VG_NAV_ABBOX 0
 #define VG_NAV_SIMPLE 1
+#define VG_NAV_ABBOX_MIXED 2
 
 /* Largest geometry depth the device navigator handles. */
 #define VG_NAV_MAX_DEPTH 16

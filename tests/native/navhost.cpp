@@ -5,7 +5,7 @@
 // on the geometry records vg_geometry_upload builds (BuildGeometry), so the
 // CPU tests can compare the navigator's logic with the oracle before any
 // GPU run.  Exports the oracle's navigation entry points with the prefix
-// navhost_ (oracle/vgo_nav.cpp has the argument meanings); navigator kind 2
+// navhost_ (oracle/vgo_nav.cpp has the argument meanings); navigator kind 9
 // runs the ABBox navigator with only its exact sorted-order candidate walk.
 #include <thread>
 #include <vector>
@@ -16,6 +16,9 @@ using namespace vga;
 using namespace vga::nav;
 
 namespace {
+
+// Host-only navigator code: the ABBox walk in exact sorted order alone.
+constexpr int kExactOnlyKind = 9;
 
 struct HostGeom {
   vg_geometry g;
@@ -75,7 +78,7 @@ void Step(const HostGeom *h, long n, const double *x, const double *y, const dou
       DPath<D> c, o;
       Load(level_in, path_in, n, i, c);
       double step = 0;
-      if (c.level > 0 && kind == 2) {
+      if (c.level > 0 && kind == kExactOnlyKind) {
         FindNextBoundaryAndStep<D, true>(h->dev, V3{x[i], y[i], z[i]}, V3{dx[i], dy[i], dz[i]}, c, o, pstep, step,
                                          VG_NAV_ABBOX);
       } else if (c.level > 0) {
@@ -113,7 +116,7 @@ void Trace(const HostGeom *h, long n, const double *x, const double *y, const do
       uint64_t hh;
       int nexact;
       const V3 dd{dx[i], dy[i], dz[i]};
-      nsteps[i] = kind == 2 ? TraceOne<D, true>(h->dev, p, dd, pstep, VG_NAV_ABBOX, max_steps, dist, hh, nexact)
+      nsteps[i] = kind == kExactOnlyKind ? TraceOne<D, true>(h->dev, p, dd, pstep, VG_NAV_ABBOX, max_steps, dist, hh, nexact)
                             : TraceOne<D>(h->dev, p, dd, pstep, kind, max_steps, dist, hh, nexact);
       fx[i] = p.x;
       fy[i] = p.y;
@@ -144,6 +147,8 @@ void *navhost_geom_create(int n_shapes, const int *shape_type, const int *shape_
   h->dev.params = h->g.params.data();
   h->dev.daughters = h->g.daughters.data();
   h->dev.aabb = h->aabbf.data();
+  h->dev.b32 = h->g.b32.data();
+  h->dev.b32side = h->g.b32side.data();
   h->dev.world = h->g.world;
   h->dev.n_placed = (int32_t)h->g.placed.size();
   h->dev.n_daughters = (int32_t)h->g.daughters.size();
@@ -227,6 +232,45 @@ void navhost_trace(void *hv, long n, const double *x, const double *y, const dou
     Trace<8>(h, n, x, y, z, dx, dy, dz, pstep, kind, max_steps, nsteps, fx, fy, fz, travelled, hash, nthreads);
   else
     Trace<16>(h, n, x, y, z, dx, dy, dz, pstep, kind, max_steps, nsteps, fx, fy, fz, travelled, hash, nthreads);
+}
+
+// FP32 filter of the mixed-precision forms (vg_bound32.hpp) on placed
+// volume `placed`: Classify32 per point (0 inside, 2 outside, -1
+// undecided), and whether each ray provably misses the extent box.
+void navhost_classify32(void *hv, int placed, long n, const float *x, const float *y, const float *z,
+                        signed char *out) {
+  const auto *h = static_cast<HostGeom *>(hv);
+  VGA_NAV_CHECK(placed >= 0 && placed < (int)h->g.b32.size(), "placed id out of range");
+  const B32 &b = h->g.b32[placed];
+  for (long i = 0; i < n; ++i) out[i] = (signed char)Classify32(b, h->g.b32side.data(), F3{x[i], y[i], z[i]});
+}
+
+void navhost_raymiss32(void *hv, int placed, long n, const float *x, const float *y, const float *z, const float *dx,
+                       const float *dy, const float *dz, unsigned char *out) {
+  const auto *h = static_cast<HostGeom *>(hv);
+  VGA_NAV_CHECK(placed >= 0 && placed < (int)h->g.b32.size(), "placed id out of range");
+  const B32 &b = h->g.b32[placed];
+  const float *e = h->g.extent32.data() + 6 * (size_t)placed;
+  // 1: miss with DistanceToIn kInfinity; 2: miss with NaN (the Orb's
+  // approaching rays); 0: undecided.  As the batch kernel's filter.
+  for (long i = 0; i < n; ++i) {
+    const F3 p{x[i], y[i], z[i]}, d{dx[i], dy[i], dz[i]};
+    int r = RayMissesBox32(b, e, p, d) ? 1 : 0;
+    if (r && b.kind == VG_ORB) {
+      const int sg = RadialSign32(b, p, d);
+      r = sg == 0 ? 0 : (sg < 0 ? 2 : 1);
+    }
+    out[i] = (unsigned char)r;
+  }
+}
+
+// so, si and the margin of each point (for checks of the bounds themselves).
+void navhost_bound32(void *hv, int placed, long n, const float *x, const float *y, const float *z, float *so,
+                     float *si, float *m) {
+  const auto *h = static_cast<HostGeom *>(hv);
+  VGA_NAV_CHECK(placed >= 0 && placed < (int)h->g.b32.size(), "placed id out of range");
+  const B32 &b = h->g.b32[placed];
+  for (long i = 0; i < n; ++i) Bound32(b, h->g.b32side.data(), F3{x[i], y[i], z[i]}, so[i], si[i], m[i]);
 }
 
 }  // extern "C"

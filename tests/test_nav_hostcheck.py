@@ -53,22 +53,24 @@ def test_host_build_matches_oracle(oracle, hostnav, case):
     lo, pa = O.locate(p, nthreads=8)
     lh, ph = H.locate(p, nthreads=8)
     assert (lo == lh).all() and (pa == ph).all()
-    for kind in (0, 1):
-        for x, y in zip(O.step(p, d, lo, pa, kind=kind, nthreads=8), H.step(p, d, lo, pa, kind=kind, nthreads=8)):
-            assert same(x, y).all(), kind
-        for x, y in zip(O.trace(p, d, kind=kind, nthreads=8), H.trace(p, d, kind=kind, nthreads=8)):
-            assert same(x, y).all(), kind
+    # kind 2 (the mixed-precision ABBox navigator) must equal the oracle's
+    # ABBox navigator (kind 0) bit for bit.
+    for kind, hkind in ((0, 0), (1, 1), (0, 2)):
+        for x, y in zip(O.step(p, d, lo, pa, kind=kind, nthreads=8), H.step(p, d, lo, pa, kind=hkind, nthreads=8)):
+            assert same(x, y).all(), hkind
+        for x, y in zip(O.trace(p, d, kind=kind, nthreads=8), H.trace(p, d, kind=hkind, nthreads=8)):
+            assert same(x, y).all(), hkind
     assert same(O.safety(p, lo, pa, nthreads=8), H.safety(p, lo, pa, nthreads=8)).all()
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_exact_walk_matches_oracle(oracle, hostnav, case):
     """The ABBox candidate walk in exact sorted order alone (navigator kind
-    2 of the host build) agrees with the oracle's sorted walk."""
+    9 of the host build) agrees with the oracle's sorted walk."""
     flat, p, d = CASES[case]
     O, H = po.Nav(oracle, flat), po.Nav(hostnav, flat)
     lo, pa = O.locate(p, nthreads=8)
-    for x, y in zip(O.step(p, d, lo, pa, kind=0, nthreads=8), H.step(p, d, lo, pa, kind=2, nthreads=8)):
+    for x, y in zip(O.step(p, d, lo, pa, kind=0, nthreads=8), H.step(p, d, lo, pa, kind=9, nthreads=8)):
         assert same(x, y).all()
-    for x, y in zip(O.trace(p, d, kind=0, nthreads=8), H.trace(p, d, kind=2, nthreads=8)):
+    for x, y in zip(O.trace(p, d, kind=0, nthreads=8), H.trace(p, d, kind=9, nthreads=8)):
         assert same(x, y).all()

@@ -1,9 +1,12 @@
-"""Tasks, batches and FP32-storage queries through the C-ABI.
+"""Tasks, batches, FP32-storage and mixed-precision queries through the C-ABI.
 
 A task is one basket query on one placed volume (VPlacedVolume SOA3D
 virtuals, volumes/PlacedVolume.h:147-246) whose inputs may be float64 (the
-FP64 form) or float32 (FP32 storage, promoted to double before any
-arithmetic).  A Batch runs a fixed table of tasks -- any mix of shapes,
+FP64 form) or float32: FP32 storage (promoted to double before any
+arithmetic) or, with mixed=True, the mixed-precision form (an FP32 filter
+decides Contains / Inside and DistanceToIn misses with a rigorous rounding
+margin; every other lane runs the FP64 code, so results equal FP32
+storage's bit for bit).  A Batch runs a fixed table of tasks -- any mix of shapes,
 queries and basket sizes -- in one kernel launch per shape weight class, and
 can autotune its launch shape.  Work goes on torch's current stream.
 """
@@ -14,7 +17,7 @@ import torch
 from . import capi
 
 CONTAINS, INSIDE, DISTANCE_TO_IN, DISTANCE_TO_OUT, PLACED_DISTANCE_TO_OUT, SAFETY_TO_IN, SAFETY_TO_OUT = range(7)
-F64, F32 = 0, 1
+F64, F32, MIXED = 0, 1, 2
 OUT_DTYPE = {CONTAINS: torch.uint8, INSIDE: torch.int32}
 DIRECTIONAL = {DISTANCE_TO_IN, DISTANCE_TO_OUT, PLACED_DISTANCE_TO_OUT}
 
@@ -32,6 +35,7 @@ class CQueryTask(ctypes.Structure):
         ("dx", c_vp), ("dy", c_vp), ("dz", c_vp),
         ("step_max", c_vp),
         ("out", c_vp),
+        ("fp64_count", c_vp),
     ]
 
 
@@ -61,9 +65,12 @@ def _stream():
 
 class Task:
     """One query: placed id, query code, point triple, optional direction
-    triple and step_max, output tensor (allocated if None)."""
+    triple and step_max, output tensor (allocated if None).  mixed=True
+    (float32 inputs only) selects the mixed-precision form; fp64_count, a
+    one-element int64 CUDA tensor, receives the number of queries each run
+    evaluated in FP64."""
 
-    def __init__(self, placed, query, p, d=None, step_max=None, out=None):
+    def __init__(self, placed, query, p, d=None, step_max=None, out=None, mixed=False, fp64_count=None):
         if len(p) != 3 or (query in DIRECTIONAL and (d is None or len(d) != 3)):
             raise ValueError("points (and directions for distance queries) must be (x, y, z) triples")
         if query not in range(7):
@@ -85,14 +92,21 @@ class Task:
         elif (out.dtype != odt or out.numel() != n or not out.is_cuda or out.device != dev
               or not out.is_contiguous()):
             raise ValueError(f"out must be a contiguous {odt} CUDA tensor with one entry per point")
+        if mixed and dt != torch.float32:
+            raise ValueError("the mixed-precision form takes float32 inputs")
+        if fp64_count is not None and (fp64_count.dtype != torch.int64 or fp64_count.numel() < 1
+                                       or not fp64_count.is_cuda or fp64_count.device != dev):
+            raise ValueError("fp64_count must be an int64 CUDA tensor on the inputs' device")
         self.placed, self.query, self.p, self.d, self.step_max, self.out, self.n = placed, query, p, d, step_max, out, n
-        self.input_type = F32 if dt == torch.float32 else F64
+        self.fp64_count = fp64_count
+        self.input_type = MIXED if mixed else (F32 if dt == torch.float32 else F64)
 
     def c(self):
         d = self.d if self.d is not None else (None, None, None)
         ptr = lambda a: None if a is None else a.data_ptr()
         return CQueryTask(int(self.placed), int(self.query), self.input_type, 0, self.n,
-                          *[ptr(a) for a in self.p], *[ptr(a) for a in d], ptr(self.step_max), self.out.data_ptr())
+                          *[ptr(a) for a in self.p], *[ptr(a) for a in d], ptr(self.step_max), self.out.data_ptr(),
+                          ptr(self.fp64_count))
 
 
 def query(geom, task):

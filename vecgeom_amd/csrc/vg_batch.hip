@@ -27,7 +27,13 @@
 // Inputs may be double (the FP64 form) or float (FP32 storage: each value is
 // promoted to double in registers, so results equal the FP64 kernels on the
 // same float-valued inputs; distances keep FP64 arithmetic, which the
-// 1e-9 mm bar requires -- one float ulp at 50 mm is 4e-6 mm).
+// 1e-9 mm bar requires -- one float ulp at 50 mm is 4e-6 mm).  Mixed tasks
+// (float inputs) run an FP32 filter first (vg_bound32.hpp): Contains and
+// Inside are decided in FP32 where the point lies farther from the surface
+// than the bound's rigorous rounding margin, and DistanceToIn returns
+// kInfinity for rays that provably miss the shape's extent box; every other
+// lane runs the FP64 code on the promoted inputs, so results equal the FP32
+// storage form's bit for bit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,8 +60,46 @@ struct TaskDev {
   const void *x, *y, *z, *dx, *dy, *dz;
   const double *sm;
   void *out;
+  uint64_t *count;      // FP64-evaluated queries (nullable)
+  const B32 *b32;       // the placed volume's FP32 bound record
+  const float *b32side; // its variable-length data
+  const float *ext;     // its local extent box, lo xyz, hi xyz (float)
   Xf xf;
 };
+
+// Adds each lane's n to *count with one atomic per wave; callable with any
+// set of active lanes (the sum is taken bit by bit over ballots).
+__device__ __forceinline__ void WaveCount(uint64_t *count, uint32_t n) {
+  uint64_t total = 0;
+  for (int b = 0; b < 32; ++b) {
+    total += (uint64_t)__popcll(__ballot((n >> b) & 1u)) << b;
+    if (__ballot((n >> b) > 1u) == 0) break;
+  }
+  const uint64_t act = __ballot(1);
+  if (total && (uint32_t)__lane_id() == (uint32_t)(__ffsll((long long)act) - 1)) atomicAdd((unsigned long long *)count, total);
+}
+
+// FP32 filter of a mixed task for query Q at float point p (direction d):
+// true with the result in r when decided.
+template <int Q>
+__device__ __forceinline__ bool Filter32(const TaskDev &t, F3 p, F3 d, double &r) {
+  if constexpr (Q == kQDistIn) {
+    if (!RayMissesBox32(*t.b32, t.ext, p, d)) return false;
+    if (t.type == VG_ORB) {  // the Orb's miss value depends on the ray's sense
+      const int sg = RadialSign32(*t.b32, p, d);
+      if (sg == 0) return false;
+      r = sg < 0 ? __builtin_nan("") : kInfinity;
+      return true;
+    }
+    r = kInfinity;
+    return true;
+  } else {
+    const int c = Classify32(*t.b32, t.b32side, p);
+    if (c < 0) return false;
+    r = Q == kQContains ? (c == kInside ? 1. : 0.) : (double)c;
+    return true;
+  }
+}
 
 template <class T, int IT>
 struct alignas(sizeof(T) * IT) Vec {
@@ -78,7 +122,7 @@ __device__ __forceinline__ void Load(const void *base, uint64_t i0, bool full, b
 template <int IT>
 __device__ __forceinline__ void LoadAll(const TaskDev t, uint64_t i0, bool full, const void *a, double (&o)[IT]) {
   const bool vec = t.vec_ok != 0;
-  if (t.in_type == VG_F32)
+  if (t.in_type != VG_F64)
     Load<float, IT>(a, i0, full, vec, t.n, o);
   else
     Load<double, IT>(a, i0, full, vec, t.n, o);
@@ -101,11 +145,30 @@ __device__ __forceinline__ void RunTile(const TaskDev t, const double *blk, uint
 #pragma unroll
     for (int k = 0; k < IT; ++k) sm[k] = (t.sm && i0 + k < t.n) ? t.sm[i0 + k] : kInfinity;
   }
+  constexpr bool filt = (Q == kQContains || Q == kQInside || Q == kQDistIn);
+  const bool mixed = t.in_type == VG_MIXED;
+  uint32_t n64 = 0;
 #pragma unroll
   for (int k = 0; k < IT; ++k) {
     if (i0 + k >= t.n) break;
     const V3 p{px[k], py[k], pz[k]};
     const uint64_t i = i0 + k;
+    if constexpr (filt) {
+      if (mixed) {
+        double r;
+        const F3 d32 = dir ? F3{(float)dx[k], (float)dy[k], (float)dz[k]} : F3{0.f, 0.f, 0.f};
+        if (Filter32<Q>(t, F3{(float)p.x, (float)p.y, (float)p.z}, d32, r)) {
+          if constexpr (Q == kQContains)
+            static_cast<uint8_t *>(t.out)[i] = (uint8_t)r;
+          else if constexpr (Q == kQInside)
+            static_cast<int32_t *>(t.out)[i] = (int32_t)r;
+          else
+            static_cast<double *>(t.out)[i] = r;
+          continue;
+        }
+      }
+    }
+    ++n64;
     if constexpr (Q == kQContains) {
       static_cast<uint8_t *>(t.out)[i] = S::Contains(P, XfPointRt(t.xf, t.xfkind, p)) ? 1 : 0;
     } else if constexpr (Q == kQInside) {
@@ -126,6 +189,7 @@ __device__ __forceinline__ void RunTile(const TaskDev t, const double *blk, uint
       static_cast<double *>(t.out)[i] = S::SafetyToOut(P, p);
     }
   }
+  if (t.count) WaveCount(t.count, n64);
 }
 
 template <class S, int IT>
@@ -221,6 +285,7 @@ __device__ __forceinline__ void RunTileCompactDistIn(const TaskDev t, const doub
   const uint64_t i0 = tile_first + (uint64_t)threadIdx.x * IT;
   const bool full = i0 + IT <= t.n;
   double px[IT], py[IT], pz[IT], dx[IT], dy[IT], dz[IT];
+  uint32_t n64 = 0;
   if (i0 < t.n) {
     LoadAll<IT>(t, i0, full, t.x, px);
     LoadAll<IT>(t, i0, full, t.y, py);
@@ -232,10 +297,16 @@ __device__ __forceinline__ void RunTileCompactDistIn(const TaskDev t, const doub
     for (int k = 0; k < IT; ++k) {
       const uint64_t i = i0 + k;
       if (i >= t.n) break;
+      double r;
+      if (t.in_type == VG_MIXED && Filter32<kQDistIn>(t, F3{(float)px[k], (float)py[k], (float)pz[k]},
+                                                      F3{(float)dx[k], (float)dy[k], (float)dz[k]}, r)) {
+        static_cast<double *>(t.out)[i] = r;
+        continue;
+      }
+      ++n64;
       const V3 p = XfPointRt(t.xf, t.xfkind, V3{px[k], py[k], pz[k]});
       const V3 d = XfDirRt(t.xf, t.xfkind, V3{dx[k], dy[k], dz[k]});
       const double sm = t.sm ? t.sm[i] : kInfinity;
-      double r;
       if (S::DistanceToInQuick(P, p, d, sm, r)) {
         static_cast<double *>(t.out)[i] = r;
       } else {
@@ -251,6 +322,7 @@ __device__ __forceinline__ void RunTileCompactDistIn(const TaskDev t, const doub
       }
     }
   }
+  if (t.count) WaveCount(t.count, n64);
   __syncthreads();
   const int c = count;
   for (int j = threadIdx.x; j < c; j += kBatchBlock) {
@@ -260,7 +332,7 @@ __device__ __forceinline__ void RunTileCompactDistIn(const TaskDev t, const doub
 }
 
 __device__ __forceinline__ double LoadOne(const TaskDev &t, const void *a, uint64_t i) {
-  return t.in_type == VG_F32 ? (double)static_cast<const float *>(a)[i] : static_cast<const double *>(a)[i];
+  return t.in_type != VG_F64 ? (double)static_cast<const float *>(a)[i] : static_cast<const double *>(a)[i];
 }
 
 // Items [first, first + count) of a walk task with lane refill: each of the
@@ -276,18 +348,25 @@ __device__ __forceinline__ void RunTileWalk(const TaskDev &t, const double *blk,
   const uint64_t begin = b < lim ? b : lim;
   const uint64_t end = begin + per < lim ? begin + per : lim;
   double *out = static_cast<double *>(t.out);
+  const bool filt = !Local && t.in_type == VG_MIXED && t.query == kQDistIn;
+  uint32_t n64 = 0;
   WalkRefill<Ops>(
       ParamView<S>::Get(blk), begin, end,
-      [&](uint64_t i, V3 &p, V3 &d, double &sm) {
+      [&](uint64_t i, V3 &p, V3 &d, double &sm, double &r) {
         p = V3{LoadOne(t, t.x, i), LoadOne(t, t.y, i), LoadOne(t, t.z, i)};
         d = V3{LoadOne(t, t.dx, i), LoadOne(t, t.dy, i), LoadOne(t, t.dz, i)};
+        if (filt && Filter32<kQDistIn>(t, F3{(float)p.x, (float)p.y, (float)p.z}, F3{(float)d.x, (float)d.y, (float)d.z}, r))
+          return true;
+        ++n64;
         if (!Local) {
           p = XfPointRt(t.xf, t.xfkind, p);
           d = XfDirRt(t.xf, t.xfkind, d);
         }
         sm = t.sm ? t.sm[i] : kInfinity;
+        return false;
       },
       [&](uint64_t i, double r) { out[i] = r; });
+  if (t.count) WaveCount(t.count, n64);
 }
 
 // Copies a task's parameter block to LDS (whole workgroup).
@@ -479,7 +558,7 @@ void LaunchIT(const vg_batch *b, unsigned grid, hipStream_t s) {
 // Algorithmic bytes of one task: every input element read once, every
 // output element written once.
 uint64_t TaskBytes(const TaskDev &t) {
-  const uint64_t w = t.in_type == VG_F32 ? 4 : 8;
+  const uint64_t w = t.in_type == VG_F64 ? 8 : 4;
   const bool dir = t.query == kQDistIn || t.query == kQDistOut || t.query == kQPlacedDistOut;
   uint64_t per = 3 * w;
   if (dir) per += 3 * w + (t.sm ? 8 : 0);
@@ -609,7 +688,7 @@ int MakeTask(const vg_geometry *g, const vg_query_task &q, int i, TaskDev &t) {
   if (!ShapeHasKernels(pr.type))
     return SetError(VG_ERR_UNSUPPORTED, "task " + std::to_string(i) + ": no device kernels for the shape");
   const int qc = QueryCode(q.query);
-  if (qc < 0 || (q.input_type != VG_F64 && q.input_type != VG_F32))
+  if (qc < 0 || (q.input_type != VG_F64 && q.input_type != VG_F32 && q.input_type != VG_MIXED))
     return SetError(VG_ERR_INVALID, "task " + std::to_string(i) + ": bad query or input type");
   const bool dir = qc == kQDistIn || qc == kQDistOut || qc == kQPlacedDistOut;
   if (q.n > 0 && (!q.x || !q.y || !q.z || !q.out || (dir && (!q.dx || !q.dy || !q.dz))))
@@ -633,10 +712,14 @@ int MakeTask(const vg_geometry *g, const vg_query_task &q, int i, TaskDev &t) {
   t.dz = q.dz;
   t.sm = q.step_max;
   t.out = q.out;
+  t.count = q.fp64_count;
+  t.b32 = g->d_b32 + q.placed;
+  t.b32side = g->d_b32side;
+  t.ext = g->d_extent32 + 6 * (size_t)q.placed;
   t.xf = pr.xf;
   // Vector loads need every input array aligned to a whole vector of the
   // largest IT (4 items).
-  const size_t a = (t.in_type == VG_F32 ? 4 : 8) * 4;
+  const size_t a = (t.in_type == VG_F64 ? 8 : 4) * 4;
   bool ok = Aligned(t.x, a) && Aligned(t.y, a) && Aligned(t.z, a);
   if (dir) ok = ok && Aligned(t.dx, a) && Aligned(t.dy, a) && Aligned(t.dz, a);
   t.vec_ok = ok ? 1 : 0;

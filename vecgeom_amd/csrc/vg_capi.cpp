@@ -33,6 +33,12 @@ void FreeDevice(vg_geometry *g) {
   if (g->d_placed) (void)hipFree(g->d_placed);
   if (g->d_daughters) (void)hipFree(g->d_daughters);
   if (g->d_aabb) (void)hipFree(g->d_aabb);
+  if (g->d_b32) (void)hipFree(g->d_b32);
+  if (g->d_b32side) (void)hipFree(g->d_b32side);
+  if (g->d_extent32) (void)hipFree(g->d_extent32);
+  g->d_b32 = nullptr;
+  g->d_b32side = nullptr;
+  g->d_extent32 = nullptr;
   g->d_params = nullptr;
   g->d_placed = nullptr;
   g->d_daughters = nullptr;
@@ -102,6 +108,9 @@ int vg_geometry_upload(const vg_flat_geometry *f, vg_geometry **out) {
   if (!rc) rc = Upload(g->placed, &g->d_placed);
   if (!rc) rc = Upload(g->daughters, &g->d_daughters);
   if (!rc) rc = Upload(faabb, &g->d_aabb);
+  if (!rc) rc = Upload(g->b32, &g->d_b32);
+  if (!rc) rc = Upload(g->b32side, &g->d_b32side);
+  if (!rc) rc = Upload(g->extent32, &g->d_extent32);
   if (rc) {
     FreeDevice(g);
     delete g;

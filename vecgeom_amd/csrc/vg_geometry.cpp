@@ -150,6 +150,9 @@ int BuildGeometry(const vg_flat_geometry *f, vg_geometry *g) {
   // Placed volumes.
   g->placed.resize(f->n_placed);
   g->aabb.resize(6 * (size_t)f->n_placed);
+  g->b32.resize(f->n_placed);
+  g->extent32.resize(6 * (size_t)f->n_placed);
+  g->b32side.clear();
   for (int p = 0; p < f->n_placed; ++p) {
     const int l = f->placed_logical[p];
     if (l < 0 || l >= f->n_logical) {
@@ -172,6 +175,11 @@ int BuildGeometry(const vg_flat_geometry *f, vg_geometry *g) {
       return SetError(VG_ERR_UNSUPPORTED, "no extent for shape type " + std::to_string(r.type));
     }
     ComputeAABB(lo, hi, r.xf, g->aabb.data() + 6 * (size_t)p);
+    BuildBound32(r.type, g->params.data() + r.param_off, r.xf, lo, hi, g->b32[p], g->b32side);
+    for (int i = 0; i < 3; ++i) {
+      g->extent32[6 * (size_t)p + i] = (float)lo[i];
+      g->extent32[6 * (size_t)p + 3 + i] = (float)hi[i];
+    }
     if (!g->nav_bad_type && !ShapeHasKernels(r.type)) g->nav_bad_type = r.type;
     if (r.type >= 0 && r.type < 32) g->type_mask |= 1u << r.type;
   }

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/vecgeom_amd.h"
+#include "vg_bound32.hpp"
 #include "vg_math.hpp"
 
 namespace vga {
@@ -51,11 +52,17 @@ struct vg_geometry {
   std::vector<int32_t> logical_dbegin;  // n_logical + 1
   std::vector<int32_t> daughters;
   std::vector<double> aabb;             // 6 per placed: lo xyz, hi xyz (mother frame)
+  std::vector<vga::B32> b32;            // FP32 bound record per placed volume
+  std::vector<float> b32side;           // variable-length bound data
+  std::vector<float> extent32;          // 6 per placed: local extent lo xyz, hi xyz
   // Device copies.
   double *d_params = nullptr;
   vga::PlacedRec *d_placed = nullptr;
   int32_t *d_daughters = nullptr;
   float *d_aabb = nullptr;  // 6 floats per placed (ABBoxManager keeps floats)
+  vga::B32 *d_b32 = nullptr;
+  float *d_b32side = nullptr;
+  float *d_extent32 = nullptr;
   int32_t nav_bad_type = 0;  // first shape type in the tree without device kernels (0: none)
   uint32_t type_mask = 0;    // bit t set: some placed volume has shape type t
 };
@@ -76,6 +83,10 @@ int BuildGeometry(const vg_flat_geometry *f, vg_geometry *g);
 std::vector<double> BuildParams(int type, const double *ctor, int n, std::string *err);
 // Local-frame axis-aligned extent of a shape (UnplacedX::Extent).
 bool ShapeExtent(int type, const double *block, double lo[3], double hi[3]);
+// FP32 bound record of one placed volume (vg_bound32.cpp); variable-length
+// data is appended to `side`.
+void BuildBound32(int type, const double *blk, const Xf &xf, const double lo[3], const double hi[3], B32 &b,
+                  std::vector<float> &side);
 
 // Query kinds dispatched by vg_shape_kernels.hip.
 enum Query : int {

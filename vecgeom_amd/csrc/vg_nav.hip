@@ -180,6 +180,8 @@ NavDev Dev(const vg_geometry *g) {
   d.params = g->d_params;
   d.daughters = g->d_daughters;
   d.aabb = g->d_aabb;
+  d.b32 = g->d_b32;
+  d.b32side = g->d_b32side;
   d.world = g->world;
   d.n_placed = (int32_t)g->placed.size();
   d.n_daughters = (int32_t)g->daughters.size();

@@ -14,7 +14,7 @@ namespace {
 
 int CheckNav(const vg_geometry *g, int kind) {
   if (!g) return SetError(VG_ERR_INVALID, "null geometry");
-  if (kind != VG_NAV_ABBOX && kind != VG_NAV_SIMPLE) return SetError(VG_ERR_INVALID, "unknown navigator");
+  if (kind != VG_NAV_ABBOX && kind != VG_NAV_SIMPLE && kind != VG_NAV_ABBOX_MIXED) return SetError(VG_ERR_INVALID, "unknown navigator");
   if (g->max_depth > VG_NAV_MAX_DEPTH)
     return SetError(VG_ERR_UNSUPPORTED, "geometry depth " + std::to_string(g->max_depth) + " exceeds " +
                                             std::to_string(VG_NAV_MAX_DEPTH));

@@ -37,6 +37,8 @@ struct NavDev {
   const double *params;
   const int32_t *daughters;
   const float *aabb;
+  const B32 *b32;        // FP32 bound records (mixed-precision navigator)
+  const float *b32side;  // their variable-length data
   int32_t world;
   int32_t n_placed;
   int32_t n_daughters;
@@ -52,6 +54,10 @@ struct NavDev {
   VGA_NAV const float *Box(int id) const {
     VGA_NAV_CHECK(id >= 0 && id < n_placed, "box id out of range");
     return aabb + 6 * (size_t)id;
+  }
+  VGA_NAV const B32 &Bound(int id) const {
+    VGA_NAV_CHECK(id >= 0 && id < n_placed, "bound id out of range");
+    return b32[id];
   }
 };
 
@@ -445,17 +451,33 @@ VGA_NAV Ray MakeRay(V3 lp, V3 ld) {
 // holds the mother's DistanceToOut (kInfinity if negative).  Returns 1 if
 // the exact sorted-order walk was needed.  ExactOnly skips the order-free
 // walk (used by the host-side checks of the exact walk).
+//
+// Mixed (VG_NAV_ABBOX_MIXED): before its FP64 Contains and DistanceToIn, a
+// hit daughter is skipped when its FP32 lower bound on the distance from the
+// point (vg_bound32.hpp) exceeds the best step found so far by more than
+// the bound's rounding margin and 1e-6 mm.  Such a daughter cannot contain
+// the point, and its DistanceToIn (the ray's distance to its surface, never
+// shorter than the Euclidean distance by more than the tolerance) exceeds
+// the final step, so it can neither win nor tie: the walk's result is
+// unchanged.
 template <bool ExactOnly = false, uint32_t M = kAllShapes>
-VGA_NAV int AbboxCandidates(const NavDev &g, int db, int de, V3 lp, V3 ld, double &step, int &next) {
+VGA_NAV int AbboxCandidates(const NavDev &g, int db, int de, V3 lp, V3 ld, double &step, int &next,
+                            bool mixed = false) {
   const Ray ray = MakeRay(lp, ld);
   const double step0 = step;
   double best = step0;
   int bk = -1, ties = 0;
   bool exact = ExactOnly;
+  const F3 lp32{(float)lp.x, (float)lp.y, (float)lp.z};
   for (int k = db; k < de && !ExactOnly; ++k) {
     const int cand = g.Daughter(k);
     const float b = IntersectBox(g.Box(cand), ray);
     if (!(b < kInfF)) continue;
+    if (mixed) {
+      float so, si, m;
+      Bound32(g.Bound(cand), g.b32side, lp32, so, si, m);
+      if ((double)so - (double)m > Max(best, 0.) + 1e-6) continue;
+    }
     const PlacedRec &r = g.Pv(cand);
     Xf x;
     LoadXf(r, x);
@@ -540,9 +562,10 @@ VGA_NAV int FindNextBoundaryAndStep(const NavDev &g, V3 gp, V3 gd, const DPath<D
   int exact = 0;
   step = DDistOut<M>(C.type, g.params + C.param_off, lp, ld, pstep);
   nst = cur;
-  if (kind == VG_NAV_ABBOX) {
+  const bool abbox = kind == VG_NAV_ABBOX || kind == VG_NAV_ABBOX_MIXED;
+  if (abbox) {
     if (step < 0.) step = kInfinity;
-    if (de > db) exact = AbboxCandidates<ExactOnly, M>(g, db, de, lp, ld, step, next);
+    if (de > db) exact = AbboxCandidates<ExactOnly, M>(g, db, de, lp, ld, step, next, kind == VG_NAV_ABBOX_MIXED);
     if (step > pstep) {
       step = pstep;
       nst.boundary = 0;
@@ -587,7 +610,7 @@ VGA_NAV int FindNextBoundaryAndStep(const NavDev &g, V3 gp, V3 gd, const DPath<D
     return exact;
   }
   Relocate<D, M>(g, np, nst);
-  if (kind == VG_NAV_ABBOX) {
+  if (abbox) {
     while (nst.Top() == cv) nst.Pop();
   }
   return exact;

@@ -424,7 +424,7 @@ int vg_server_create(const vg_geometry *g, uint64_t hbm_budget_bytes, int32_t na
                      vg_server **out) {
   if (!g || !out) return SetError(VG_ERR_INVALID, "null argument");
   *out = nullptr;
-  if (navigator != VG_NAV_ABBOX && navigator != VG_NAV_SIMPLE) return SetError(VG_ERR_INVALID, "unknown navigator");
+  if (navigator != VG_NAV_ABBOX && navigator != VG_NAV_SIMPLE && navigator != VG_NAV_ABBOX_MIXED) return SetError(VG_ERR_INVALID, "unknown navigator");
   if (g->max_depth > VG_NAV_MAX_DEPTH) return SetError(VG_ERR_UNSUPPORTED, "geometry too deep for the navigator");
   if (g->nav_bad_type) return SetError(VG_ERR_UNSUPPORTED, "no device kernels for a shape of the volume tree");
   int rc = EnsureDevice();
@@ -436,8 +436,8 @@ int vg_server_create(const vg_geometry *g, uint64_t hbm_budget_bytes, int32_t na
   s->g = g;
   s->navigator = navigator;
   s->pstep = pstep;
-  s->dev = NavDev{g->d_placed, g->d_params, g->d_daughters, g->d_aabb, g->world, (int32_t)g->placed.size(),
-                  (int32_t)g->daughters.size(), 0};
+  s->dev = NavDev{g->d_placed, g->d_params, g->d_daughters, g->d_aabb, g->d_b32, g->d_b32side,
+                  g->world,    (int32_t)g->placed.size(), (int32_t)g->daughters.size(), 0};
   rc = CheckHip(hipMalloc(&s->mem, cap * SlotBytes(depth)), "hipMalloc(serve arena)");
   if (!rc) {
     char *m = static_cast<char *>(s->mem);

@@ -303,10 +303,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWalkWav
   double *out = reinterpret_cast<double *>(a.out);
   WalkRefill<Ops>(
       P, begin, end,
-      [&](uint64_t i, V3 &p, V3 &d, double &sm) {
+      [&](uint64_t i, V3 &p, V3 &d, double &sm, double &) {
         p = XfPoint<XK>(X, V3{a.x[i], a.y[i], a.z[i]});
         d = XfDir<XK>(X, V3{a.dx[i], a.dy[i], a.dz[i]});
         sm = a.sm ? a.sm[i] : kInfinity;
+        return false;
       },
       [&](uint64_t i, double r) {
         out[i] = r;

@@ -29,7 +29,9 @@ namespace vga {
 constexpr int kRefillMin = 32;
 
 // Must be called by all 64 lanes of a wave with the same [begin, end).
-// load(i, p, d, sm) reads query i (shape frame); store(i, r) writes its result.
+// load(i, p, d, sm, r) reads query i (shape frame), or returns true with its
+// result in r when it resolves it without the walk; store(i, r) writes a
+// result.
 template <class Ops, class PT, class Load, class Store>
 __device__ __forceinline__ void WalkRefill(const PT &P, uint64_t begin, uint64_t end, Load load, Store store) {
   typename Ops::State w;
@@ -45,10 +47,11 @@ __device__ __forceinline__ void WalkRefill(const PT &P, uint64_t begin, uint64_t
             next + __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
         if (i < end) {
           V3 p, d;
-          double sm;
-          load(i, p, d, sm);
+          double sm, r;
           qi = i;
-          if (Ops::Begin(P, p, d, sm, w))
+          if (load(i, p, d, sm, r))
+            store(i, r);
+          else if (Ops::Begin(P, p, d, sm, w))
             store(i, Ops::End(P, w));
           else
             active = true;

@@ -300,9 +300,9 @@ def single_shape_flat(unplaced_list, transforms, world_half=1e4):
     return gm.CloseGeometry(), ids, gm
 
 
-def single_shape_geometry(unplaced_list, transforms, world_half=1e4):
+def single_shape_flat(unplaced_list, transforms, world_half=1e4):
     """World box holding one placement per (shape, transform); returns
-    (DeviceGeometry, [placed ids]).  Used by shape benchmarks and tests."""
+    (flat geometry, [placed ids], GeoManager) on the host."""
     gm = GeoManager()
     world_lv = LogicalVolume("world", UnplacedBox(world_half, world_half, world_half), gm)
     ids = []
@@ -312,4 +312,11 @@ def single_shape_geometry(unplaced_list, transforms, world_half=1e4):
         ids.append(pv.id)
     world = world_lv.Place("world")
     gm.SetWorld(world)
-    return DeviceGeometry(gm.CloseGeometry()), ids, gm
+    return gm.CloseGeometry(), ids, gm
+
+
+def single_shape_geometry(unplaced_list, transforms, world_half=1e4):
+    """single_shape_flat uploaded: (DeviceGeometry, [placed ids], GeoManager).
+    Used by shape benchmarks and tests."""
+    flat, ids, gm = single_shape_flat(unplaced_list, transforms, world_half)
+    return DeviceGeometry(flat), ids, gm

@@ -139,6 +139,7 @@ def shape_inputs(extent, xf, n, seed, surface_fraction=0.0, surface_sampler=None
     if surface_fraction > 0 and surface_sampler is not None:
         k = int(round(surface_fraction * n))
         loc[:, :k] = surface_sampler(k, rng)
+        loc = loc[:, rng.permutation(n)]
     d = isotropic(n, rng)
     p = local_to_master(xf, loc)
     return {
@@ -147,6 +148,58 @@ def shape_inputs(extent, xf, n, seed, surface_fraction=0.0, surface_sampler=None
         "lp": master_to_local(xf, p),
         "ld": master_to_local(xf, d, direction=True),
     }
+
+
+# ------------------------------------------------------- surface-band inputs
+SURFACE_FRACTION = 0.25
+SURFACE_SIGMA = 1e-7  # mm
+
+
+def surface_sampler(extent, contains, distance_to_out, sigma=SURFACE_SIGMA, seed_points=None):
+    """A sampler of local-frame points on a shape's surface, jittered by
+    N(0, sigma^2) per coordinate: interior points (rejection-sampled
+    uniformly in the extent with `contains`) moved along isotropic directions
+    by `distance_to_out`.  contains(lp) -> bool array and
+    distance_to_out(lp, ld) -> float array take (3, m) local points (and
+    directions); any implementation of the two queries will do (the FP64
+    kernels, or the oracle)."""
+    lo, hi = extent
+
+    def sample(k, rng):
+        got = []
+        need = k
+        while need > 0:
+            m = max(4 * need, 1024)
+            cand = rng.uniform(np.asarray(lo, np.float64), np.asarray(hi, np.float64), size=(m, 3)).T.copy()
+            inside = np.asarray(contains(cand), dtype=bool)
+            cand = cand[:, inside][:, :need]
+            got.append(cand)
+            need -= cand.shape[1]
+        lp = np.concatenate(got, axis=1)
+        ld = isotropic(k, rng)
+        t = np.asarray(distance_to_out(lp, ld), np.float64)
+        t = np.where(np.isfinite(t) & (t >= 0), t, 0.0)
+        return lp + ld * t + rng.normal(0.0, sigma, size=lp.shape)
+
+    return sample
+
+
+def mixed_inputs(extent, xf, n, seed, sampler, surface_fraction=SURFACE_FRACTION):
+    """The mixed-precision workload of one placed shape: like shape_inputs,
+    with `surface_fraction` of the points on the surface (sampler above) and
+    points and directions stored as float32.  Returns dict(p32, d32 (3, n)
+    float32 master-frame arrays; p, d, lp, ld: the float32 values promoted
+    to double, and their local-frame images, for evaluating the FP64 form
+    and the oracle on exactly the same inputs).  Float rounding moves a
+    surface point by up to half a float ulp of its master coordinates
+    (about 4e-6 mm at 100 mm), which dominates the jitter."""
+    inp = shape_inputs(extent, xf, n, seed, surface_fraction, sampler)
+    p32 = inp["p"].astype(np.float32)
+    d32 = inp["d"].astype(np.float32)
+    p = p32.astype(np.float64)
+    d = d32.astype(np.float64)
+    return {"p32": p32, "d32": d32, "p": p, "d": d, "lp": master_to_local(xf, p),
+            "ld": master_to_local(xf, d, direction=True)}
 
 
 # ---------------------------------------------------------------- navigation
