@@ -81,6 +81,9 @@ void Step(const HostGeom *h, long n, const double *x, const double *y, const dou
       if (c.level > 0 && kind == kExactOnlyKind) {
         FindNextBoundaryAndStep<D, true>(h->dev, V3{x[i], y[i], z[i]}, V3{dx[i], dy[i], dz[i]}, c, o, pstep, step,
                                          VG_NAV_ABBOX);
+      } else if (c.level > 0 && kind == VG_NAV_ABBOX_MIXED) {
+        FindNextBoundaryAndStep<D, false, kAllShapes, true>(h->dev, V3{x[i], y[i], z[i]}, V3{dx[i], dy[i], dz[i]}, c,
+                                                            o, pstep, step, kind);
       } else if (c.level > 0) {
         FindNextBoundaryAndStep(h->dev, V3{x[i], y[i], z[i]}, V3{dx[i], dy[i], dz[i]}, c, o, pstep, step, kind);
       } else {
@@ -116,8 +119,11 @@ void Trace(const HostGeom *h, long n, const double *x, const double *y, const do
       uint64_t hh;
       int nexact;
       const V3 dd{dx[i], dy[i], dz[i]};
-      nsteps[i] = kind == kExactOnlyKind ? TraceOne<D, true>(h->dev, p, dd, pstep, VG_NAV_ABBOX, max_steps, dist, hh, nexact)
-                            : TraceOne<D>(h->dev, p, dd, pstep, kind, max_steps, dist, hh, nexact);
+      nsteps[i] = kind == kExactOnlyKind
+                      ? TraceOne<D, true>(h->dev, p, dd, pstep, VG_NAV_ABBOX, max_steps, dist, hh, nexact)
+                  : kind == VG_NAV_ABBOX_MIXED
+                      ? TraceOne<D, false, kAllShapes, true>(h->dev, p, dd, pstep, kind, max_steps, dist, hh, nexact)
+                      : TraceOne<D>(h->dev, p, dd, pstep, kind, max_steps, dist, hh, nexact);
       fx[i] = p.x;
       fy[i] = p.y;
       fz[i] = p.z;
@@ -149,6 +155,7 @@ void *navhost_geom_create(int n_shapes, const int *shape_type, const int *shape_
   h->dev.aabb = h->aabbf.data();
   h->dev.b32 = h->g.b32.data();
   h->dev.b32side = h->g.b32side.data();
+  h->dev.slot_box = h->g.slot_aabb.data();
   h->dev.world = h->g.world;
   h->dev.n_placed = (int32_t)h->g.placed.size();
   h->dev.n_daughters = (int32_t)h->g.daughters.size();
@@ -232,6 +239,16 @@ void navhost_trace(void *hv, long n, const double *x, const double *y, const dou
     Trace<8>(h, n, x, y, z, dx, dy, dz, pstep, kind, max_steps, nsteps, fx, fy, fz, travelled, hash, nthreads);
   else
     Trace<16>(h, n, x, y, z, dx, dy, dz, pstep, kind, max_steps, nsteps, fx, fy, fz, travelled, hash, nthreads);
+}
+
+// The ABBoxManager box (mother frame, double, before rounding to float) and
+// the local extent BuildGeometry computed for placed volume `placed`.
+void navhost_aabb(void *hv, int placed, double *box6, double *lo, double *hi) {
+  const auto *h = static_cast<HostGeom *>(hv);
+  VGA_NAV_CHECK(placed >= 0 && placed < (int)h->g.placed.size(), "placed id out of range");
+  for (int i = 0; i < 6; ++i) box6[i] = h->g.aabb[6 * (size_t)placed + i];
+  const PlacedRec &r = h->g.placed[placed];
+  ShapeExtent(r.type, h->g.params.data() + r.param_off, lo, hi);
 }
 
 // FP32 filter of the mixed-precision forms (vg_bound32.hpp) on placed

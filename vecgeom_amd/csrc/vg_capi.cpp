@@ -36,6 +36,8 @@ void FreeDevice(vg_geometry *g) {
   if (g->d_b32) (void)hipFree(g->d_b32);
   if (g->d_b32side) (void)hipFree(g->d_b32side);
   if (g->d_extent32) (void)hipFree(g->d_extent32);
+  if (g->d_slot_aabb) (void)hipFree(g->d_slot_aabb);
+  g->d_slot_aabb = nullptr;
   g->d_b32 = nullptr;
   g->d_b32side = nullptr;
   g->d_extent32 = nullptr;
@@ -111,6 +113,7 @@ int vg_geometry_upload(const vg_flat_geometry *f, vg_geometry **out) {
   if (!rc) rc = Upload(g->b32, &g->d_b32);
   if (!rc) rc = Upload(g->b32side, &g->d_b32side);
   if (!rc) rc = Upload(g->extent32, &g->d_extent32);
+  if (!rc) rc = Upload(g->slot_aabb, &g->d_slot_aabb);
   if (rc) {
     FreeDevice(g);
     delete g;

@@ -183,6 +183,11 @@ int BuildGeometry(const vg_flat_geometry *f, vg_geometry *g) {
     if (!g->nav_bad_type && !ShapeHasKernels(r.type)) g->nav_bad_type = r.type;
     if (r.type >= 0 && r.type < 32) g->type_mask |= 1u << r.type;
   }
+  // The daughters' float boxes in daughter-slot order: the navigator's box
+  // loop then reads slot k's box without first reading slot k's id.
+  g->slot_aabb.assign(8 * g->daughters.size(), 0.f);
+  for (size_t k = 0; k < g->daughters.size(); ++k)
+    for (int i = 0; i < 6; ++i) g->slot_aabb[8 * k + i] = (float)g->aabb[6 * (size_t)g->daughters[k] + i];
   g->max_depth = Depth(g, g->world, 1, 64);
   if (g->max_depth < 0) {
     return SetError(VG_ERR_INVALID, "volume tree deeper than 64 levels or cyclic");

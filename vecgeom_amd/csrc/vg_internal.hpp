@@ -55,6 +55,7 @@ struct vg_geometry {
   std::vector<vga::B32> b32;            // FP32 bound record per placed volume
   std::vector<float> b32side;           // variable-length bound data
   std::vector<float> extent32;          // 6 per placed: local extent lo xyz, hi xyz
+  std::vector<float> slot_aabb;         // 8 per daughter slot: its float AABB, lo xyz, hi xyz, 2 unused
   // Device copies.
   double *d_params = nullptr;
   vga::PlacedRec *d_placed = nullptr;
@@ -63,6 +64,7 @@ struct vg_geometry {
   vga::B32 *d_b32 = nullptr;
   float *d_b32side = nullptr;
   float *d_extent32 = nullptr;
+  float *d_slot_aabb = nullptr;
   int32_t nav_bad_type = 0;  // first shape type in the tree without device kernels (0: none)
   uint32_t type_mask = 0;    // bit t set: some placed volume has shape type t
 };

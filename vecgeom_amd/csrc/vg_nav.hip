@@ -102,7 +102,7 @@ struct StepArgs {
   int32_t pad;
 };
 
-template <int D, uint32_t M>
+template <int D, uint32_t M, bool MX>
 __global__ __launch_bounds__(kNavBlock) void StepKernel(const NavDev g, const StepArgs a, const vg_nav_states cur,
                                                         const vg_nav_states nxt) {
   const uint64_t i = (uint64_t)blockIdx.x * kNavBlock + threadIdx.x;
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kNavBlock) void StepKernel(const NavDev g, const St
   double step = 0;
   const double ps = a.pstep ? a.pstep[i] : kInfinity;
   if (ValidState(g, c)) {
-    FindNextBoundaryAndStep<D, false, M>(g, V3{a.x[i], a.y[i], a.z[i]}, V3{a.dx[i], a.dy[i], a.dz[i]}, c, o, ps,
+    FindNextBoundaryAndStep<D, false, M, MX>(g, V3{a.x[i], a.y[i], a.z[i]}, V3{a.dx[i], a.dy[i], a.dz[i]}, c, o, ps,
                                          step, a.kind);
   } else {
     // A state outside the world has no volume to step in and stays outside
@@ -154,7 +154,7 @@ struct TraceArgs {
 // kernels measured no better with a floor.
 constexpr int TraceWaves(uint32_t m) { return m == kBasicShapes ? 5 : 1; }
 
-template <int D, uint32_t M>
+template <int D, uint32_t M, bool MX>
 __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(TraceWaves(M)))) void TraceKernel(
     const NavDev g, const TraceArgs a) {
   const uint64_t i = (uint64_t)blockIdx.x * kNavBlock + threadIdx.x;
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(Trace
   double dist;
   uint64_t h;
   int nexact;
-  const int k = TraceOne<D, false, M>(g, p, d, a.pstep, a.kind, a.max_steps, dist, h, nexact);
+  const int k = TraceOne<D, false, M, MX>(g, p, d, a.pstep, a.kind, a.max_steps, dist, h, nexact);
   a.nsteps[i] = k;
   a.fx[i] = p.x;
   a.fy[i] = p.y;
@@ -182,6 +182,7 @@ NavDev Dev(const vg_geometry *g) {
   d.aabb = g->d_aabb;
   d.b32 = g->d_b32;
   d.b32side = g->d_b32side;
+  d.slot_box = g->d_slot_aabb;
   d.world = g->world;
   d.n_placed = (int32_t)g->placed.size();
   d.n_daughters = (int32_t)g->daughters.size();
@@ -205,7 +206,8 @@ int NavStep(const vg_geometry *g, int kind, const double *x, const double *y, co
             const vg_nav_states &nxt, double *step, hipStream_t st) {
   if (n == 0) return VG_OK;
   StepArgs a{x, y, z, dx, dy, dz, pstep, step, n, kind, 0};
-  VGA_NAV_VARIANTS(g, StepKernel, dim3(Blocks(n)), dim3(kNavBlock), 0, st, Dev(g), a, cur, nxt);
+  VGA_NAV_VARIANTS_MX(g, kind == VG_NAV_ABBOX_MIXED, StepKernel, dim3(Blocks(n)), dim3(kNavBlock), 0, st, Dev(g), a,
+                      cur, nxt);
   return CheckHip(hipGetLastError(), "step launch");
 }
 
@@ -221,7 +223,7 @@ int NavTrace(const vg_geometry *g, int kind, const double *x, const double *y, c
              double *fy, double *fz, double *travelled, uint64_t *hash, int32_t *exact, hipStream_t st) {
   if (n == 0) return VG_OK;
   TraceArgs a{x, y, z, dx, dy, dz, nsteps, fx, fy, fz, travelled, hash, exact, n, pstep, kind, max_steps};
-  VGA_NAV_VARIANTS(g, TraceKernel, dim3(Blocks(n)), dim3(kNavBlock), 0, st, Dev(g), a);
+  VGA_NAV_VARIANTS_MX(g, kind == VG_NAV_ABBOX_MIXED, TraceKernel, dim3(Blocks(n)), dim3(kNavBlock), 0, st, Dev(g), a);
   return CheckHip(hipGetLastError(), "trace launch");
 }
 

@@ -39,6 +39,7 @@ struct NavDev {
   const float *aabb;
   const B32 *b32;        // FP32 bound records (mixed-precision navigator)
   const float *b32side;  // their variable-length data
+  const float *slot_box; // daughter boxes in slot order, 8 floats each
   int32_t world;
   int32_t n_placed;
   int32_t n_daughters;
@@ -54,6 +55,10 @@ struct NavDev {
   VGA_NAV const float *Box(int id) const {
     VGA_NAV_CHECK(id >= 0 && id < n_placed, "box id out of range");
     return aabb + 6 * (size_t)id;
+  }
+  VGA_NAV const float *SlotBox(int k) const {
+    VGA_NAV_CHECK(k >= 0 && k < n_daughters, "box slot out of range");
+    return slot_box + 8 * (size_t)k;
   }
   VGA_NAV const B32 &Bound(int id) const {
     VGA_NAV_CHECK(id >= 0 && id < n_placed, "bound id out of range");
@@ -305,14 +310,23 @@ VGA_NAV Xf TopMatrix(const NavDev &g, const DPath<D> &s) {
 }
 
 // Contains of a placed volume on a point of its mother frame; the local
-// point is returned through `local`.
+// point is returned through `local` (meaningful when the result is true).
+// Mixed: the FP32 filter decides first (vg_bound32.hpp: a point farther
+// from the surface than the bound's rounding margin is classified exactly
+// as the FP64 Contains classifies it), and the FP64 Contains runs only for
+// points it leaves undecided.
 template <uint32_t M = kAllShapes>
-VGA_NAV bool PlacedContains(const NavDev &g, int pv, V3 p, V3 &local) {
+VGA_NAV bool PlacedContains(const NavDev &g, int pv, V3 p, V3 &local, bool mixed = false) {
+  int c = -1;
+  if (mixed) {
+    c = Classify32(g.Bound(pv), g.b32side, F3{(float)p.x, (float)p.y, (float)p.z});
+    if (c == kOutside) return false;
+  }
   const PlacedRec &r = g.Pv(pv);
   Xf x;
   LoadXf(r, x);
   local = XfPointRt(x, r.xfkind, p);
-  return DContains<M>(r.type, g.params + r.param_off, local);
+  return c == kInside || DContains<M>(r.type, g.params + r.param_off, local);
 }
 
 template <uint32_t M = kAllShapes>
@@ -325,7 +339,7 @@ VGA_NAV double PlacedDistIn(const NavDev &g, int pv, V3 p, V3 d, double sm) {
 
 // ----------------------------------------------------------- locate family
 template <int D, uint32_t M = kAllShapes>
-VGA_NAV int LocatePoint(const NavDev &g, int vol, V3 point, DPath<D> &path, bool top) {
+VGA_NAV int LocatePoint(const NavDev &g, int vol, V3 point, DPath<D> &path, bool top, bool mixed = false) {
   int cand = vol;
   V3 tmp = point;
   if (top) {
@@ -342,7 +356,7 @@ VGA_NAV int LocatePoint(const NavDev &g, int vol, V3 point, DPath<D> &path, bool
       for (int k = db; k < de; ++k) {
         const int nx = g.Daughter(k);
         V3 tp;
-        if (PlacedContains<M>(g, nx, tmp, tp)) {
+        if (PlacedContains<M>(g, nx, tmp, tp, mixed)) {
           path.Push(nx);
           tmp = tp;
           cand = nx;
@@ -356,7 +370,7 @@ VGA_NAV int LocatePoint(const NavDev &g, int vol, V3 point, DPath<D> &path, bool
 }
 
 template <int D, uint32_t M = kAllShapes>
-VGA_NAV int LocatePointExcl(const NavDev &g, int vol, int excluded, V3 point, DPath<D> &path) {
+VGA_NAV int LocatePointExcl(const NavDev &g, int vol, int excluded, V3 point, DPath<D> &path, bool mixed = false) {
   int cand = vol;
   V3 tmp = point;
   path.Push(cand);
@@ -368,7 +382,7 @@ VGA_NAV int LocatePointExcl(const NavDev &g, int vol, int excluded, V3 point, DP
     for (int k = de - 1; k >= db; --k) {
       const int nx = g.Daughter(k);
       V3 tp;
-      if (nx != excluded && PlacedContains<M>(g, nx, tmp, tp)) {
+      if (nx != excluded && PlacedContains<M>(g, nx, tmp, tp, mixed)) {
         path.Push(nx);
         tmp = tp;
         cand = nx;
@@ -381,7 +395,7 @@ VGA_NAV int LocatePointExcl(const NavDev &g, int vol, int excluded, V3 point, DP
 }
 
 template <int D, uint32_t M = kAllShapes>
-VGA_NAV int Relocate(const NavDev &g, V3 localpoint, DPath<D> &path) {
+VGA_NAV int Relocate(const NavDev &g, V3 localpoint, DPath<D> &path, bool mixed = false) {
   int mother = path.Top();
   if (mother >= 0) {
     V3 tmp = localpoint;
@@ -396,7 +410,7 @@ VGA_NAV int Relocate(const NavDev &g, V3 localpoint, DPath<D> &path) {
     }
     if (mother >= 0) {
       path.Pop();
-      return LocatePointExcl<D, M>(g, mother, mother, tmp, path);
+      return LocatePointExcl<D, M>(g, mother, mother, tmp, path, mixed);
     }
   }
   return mother;
@@ -460,44 +474,60 @@ VGA_NAV Ray MakeRay(V3 lp, V3 ld) {
 // shorter than the Euclidean distance by more than the tolerance) exceeds
 // the final step, so it can neither win nor tie: the walk's result is
 // unchanged.
-template <bool ExactOnly = false, uint32_t M = kAllShapes>
-VGA_NAV int AbboxCandidates(const NavDev &g, int db, int de, V3 lp, V3 ld, double &step, int &next,
-                            bool mixed = false) {
+//
+// In the mixed form the walk runs in two phases per group of up to 64
+// daughters: the box tests alone (a short loop whose loads do not depend on
+// each other), recording hits in a bit mask, then the hit daughters in index
+// order -- the order of the single loop of the FP64 form, which measured
+// faster there.
+template <bool ExactOnly = false, uint32_t M = kAllShapes, bool Mixed = false>
+VGA_NAV int AbboxCandidates(const NavDev &g, int db, int de, V3 lp, V3 ld, double &step, int &next) {
   const Ray ray = MakeRay(lp, ld);
   const double step0 = step;
   double best = step0;
   int bk = -1, ties = 0;
   bool exact = ExactOnly;
   const F3 lp32{(float)lp.x, (float)lp.y, (float)lp.z};
-  for (int k = db; k < de && !ExactOnly; ++k) {
+  // One hit daughter (slot k, box entry b); false ends the walk (exact path).
+  auto visit = [&](int k, float b) -> bool {
     const int cand = g.Daughter(k);
-    const float b = IntersectBox(g.Box(cand), ray);
-    if (!(b < kInfF)) continue;
-    if (mixed) {
+    if (Mixed) {
       float so, si, m;
       Bound32(g.Bound(cand), g.b32side, lp32, so, si, m);
-      if ((double)so - (double)m > Max(best, 0.) + 1e-6) continue;
+      if ((double)so - (double)m > Max(best, 0.) + 1e-6) return true;
     }
     const PlacedRec &r = g.Pv(cand);
     Xf x;
     LoadXf(r, x);
     const V3 tp = XfPointRt(x, r.xfkind, lp);
     const double *blk = g.params + r.param_off;
-    if (b < 0.f && DContains<M>(r.type, blk, tp)) {
-      exact = true;
-      break;
-    }
+    if (b < 0.f && DContains<M>(r.type, blk, tp)) return false;
     const double dd = DDistIn<M>(r.type, blk, tp, XfDirRt(x, r.xfkind, ld), step0);
-    if (dd < 0. || dd < (double)b) {
-      exact = true;
-      break;
-    }
+    if (dd < 0. || dd < (double)b) return false;
     if (dd < best) {
       best = dd;
       bk = k - db;
       ties = 1;
     } else if (dd == best) {
       ++ties;
+    }
+    return true;
+  };
+  if constexpr (!ExactOnly && Mixed) {
+    for (int base = db; base < de && !exact; base += 64) {
+      const int cnt = de - base < 64 ? de - base : 64;
+      uint64_t hits = 0;
+      for (int j = 0; j < cnt; ++j) hits |= (uint64_t)(IntersectBox(g.SlotBox(base + j), ray) < kInfF) << j;
+      while (hits && !exact) {
+        const int k = base + __builtin_ctzll(hits);
+        hits &= hits - 1;
+        exact = !visit(k, IntersectBox(g.SlotBox(k), ray));
+      }
+    }
+  } else if constexpr (!ExactOnly) {
+    for (int k = db; k < de && !exact; ++k) {
+      const float b = IntersectBox(g.SlotBox(k), ray);
+      if (b < kInfF) exact = !visit(k, b);
     }
   }
   if (!exact && ties <= 1) {
@@ -515,7 +545,7 @@ VGA_NAV int AbboxCandidates(const NavDev &g, int db, int de, V3 lp, V3 ld, doubl
     float bb = kInfF;
     int kk = -1;
     for (int k = db; k < de; ++k) {
-      const float b = IntersectBox(g.Box(g.Daughter(k)), ray);
+      const float b = IntersectBox(g.SlotBox(k), ray);
       if (!(b < kInfF)) continue;
       const int kl = k - db;
       const bool after = (b > lastb) || (b == lastb && kl > lastk);
@@ -549,7 +579,9 @@ VGA_NAV int AbboxCandidates(const NavDev &g, int db, int de, V3 lp, V3 ld, doubl
 
 // FindNextBoundaryAndStep of both navigators (kind 0 ABBox, 1 Simple).
 // Returns 1 if the ABBox walk took the exact path.
-template <int D, bool ExactOnly = false, uint32_t M = kAllShapes>
+// Mixed: the navigator of VG_NAV_ABBOX_MIXED (kind must then be that
+// navigator): FP32 filters in the daughter walk and in the locate scans.
+template <int D, bool ExactOnly = false, uint32_t M = kAllShapes, bool Mixed = false>
 VGA_NAV int FindNextBoundaryAndStep(const NavDev &g, V3 gp, V3 gd, const DPath<D> &cur, DPath<D> &nst,
                                        double pstep, double &step, int kind) {
   const Xf m = TopMatrix(g, cur);
@@ -562,10 +594,11 @@ VGA_NAV int FindNextBoundaryAndStep(const NavDev &g, V3 gp, V3 gd, const DPath<D
   int exact = 0;
   step = DDistOut<M>(C.type, g.params + C.param_off, lp, ld, pstep);
   nst = cur;
+  constexpr bool mixed = Mixed;
   const bool abbox = kind == VG_NAV_ABBOX || kind == VG_NAV_ABBOX_MIXED;
   if (abbox) {
     if (step < 0.) step = kInfinity;
-    if (de > db) exact = AbboxCandidates<ExactOnly, M>(g, db, de, lp, ld, step, next, kind == VG_NAV_ABBOX_MIXED);
+    if (de > db) exact = AbboxCandidates<ExactOnly, M, Mixed>(g, db, de, lp, ld, step, next);
     if (step > pstep) {
       step = pstep;
       nst.boundary = 0;
@@ -606,10 +639,10 @@ VGA_NAV int FindNextBoundaryAndStep(const NavDev &g, V3 gp, V3 gd, const DPath<D
     const int nv = g.Daughter(db + next);
     Xf x;
     LoadXf(g.Pv(nv), x);
-    LocatePoint<D, M>(g, nv, ApplyGeneric(x, np), nst, false);
+    LocatePoint<D, M>(g, nv, ApplyGeneric(x, np), nst, false, mixed);
     return exact;
   }
-  Relocate<D, M>(g, np, nst);
+  Relocate<D, M>(g, np, nst, mixed);
   if (abbox) {
     while (nst.Top() == cv) nst.Pop();
   }
@@ -637,19 +670,19 @@ VGA_NAV double GetSafety(const NavDev &g, V3 gp, const DPath<D> &cur) {
 // state leaves the world or max_steps steps were taken.  Returns the number
 // of steps; p ends at the final point.  The hash folds
 // (new top id + 1) * 2 + boundary after every step.
-template <int D, bool ExactOnly = false, uint32_t M = kAllShapes>
+template <int D, bool ExactOnly = false, uint32_t M = kAllShapes, bool Mixed = false>
 VGA_NAV int TraceOne(const NavDev &g, V3 &p, V3 d, double pstep, int kind, int max_steps, double &dist, uint64_t &h,
                      int &nexact) {
   DPath<D> cur, nst;
   cur.Clear();
-  LocatePoint<D, M>(g, g.world, p, cur, true);
+  LocatePoint<D, M>(g, g.world, p, cur, true, Mixed);
   int k = 0;
   nexact = 0;
   dist = 0;
   h = 1469598103934665603ULL;
   while (cur.level > 0 && k < max_steps) {
     double step;
-    nexact += FindNextBoundaryAndStep<D, ExactOnly, M>(g, p, d, cur, nst, pstep, step, kind);
+    nexact += FindNextBoundaryAndStep<D, ExactOnly, M, Mixed>(g, p, d, cur, nst, pstep, step, kind);
     dist += step;
     const double s = step + 1E-6;
     p = V3{p.x + d.x * s, p.y + d.y * s, p.z + d.z * s};
@@ -663,6 +696,35 @@ VGA_NAV int TraceOne(const NavDev &g, V3 &p, V3 d, double pstep, int kind, int m
 
 }  // namespace nav
 }  // namespace vga
+
+// Launches KERNEL<D, M, MX> with the path depth D (8 or 16) that holds the
+// geometry, the shape set M (the basic shapes only when the geometry uses
+// nothing else, all shapes otherwise) and MX = the mixed-precision
+// navigator (MIXED, a run-time bool).
+#define VGA_NAV_VARIANTS_MX(G, MIXED, KERNEL, ...)                                        \
+  do {                                                                                    \
+    const bool light_ = ((G)->type_mask & ~::vga::nav::kBasicShapes) == 0;                \
+    const bool d8_ = (G)->max_depth <= 8;                                                 \
+    if (MIXED) {                                                                          \
+      if (d8_ && light_)                                                                  \
+        hipLaunchKernelGGL((KERNEL<8, ::vga::nav::kBasicShapes, true>), __VA_ARGS__);     \
+      else if (d8_)                                                                       \
+        hipLaunchKernelGGL((KERNEL<8, ::vga::nav::kAllShapes, true>), __VA_ARGS__);       \
+      else if (light_)                                                                    \
+        hipLaunchKernelGGL((KERNEL<16, ::vga::nav::kBasicShapes, true>), __VA_ARGS__);    \
+      else                                                                                \
+        hipLaunchKernelGGL((KERNEL<16, ::vga::nav::kAllShapes, true>), __VA_ARGS__);      \
+    } else {                                                                              \
+      if (d8_ && light_)                                                                  \
+        hipLaunchKernelGGL((KERNEL<8, ::vga::nav::kBasicShapes, false>), __VA_ARGS__);    \
+      else if (d8_)                                                                       \
+        hipLaunchKernelGGL((KERNEL<8, ::vga::nav::kAllShapes, false>), __VA_ARGS__);      \
+      else if (light_)                                                                    \
+        hipLaunchKernelGGL((KERNEL<16, ::vga::nav::kBasicShapes, false>), __VA_ARGS__);   \
+      else                                                                                \
+        hipLaunchKernelGGL((KERNEL<16, ::vga::nav::kAllShapes, false>), __VA_ARGS__);     \
+    }                                                                                     \
+  } while (0)
 
 // Launches KERNEL<D, M> with the path depth D (8 or 16) that holds the
 // geometry and the shape set M: the basic shapes only when the geometry uses

@@ -101,7 +101,7 @@ __global__ __launch_bounds__(kServeBlock) void ServeLocateKernel(const NavDev g,
 // One step of every track of every segment; a track outside the world
 // (level 0) stays where it is with step 0.  Tiles of kServeBlock tracks are
 // numbered across segments; each block finds its segment by binary search.
-template <int D, uint32_t M>
+template <int D, uint32_t M, bool MX>
 __global__ __launch_bounds__(kServeBlock) void ServeStepKernel(const NavDev g, const Arena a,
                                                                const Seg *__restrict__ segs, int nseg, uint64_t ntiles,
                                                                double pstep, int kind) {
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kServeBlock) void ServeStepKernel(const NavDev g, c
   V3 p{a.x[s], a.y[s], a.z[s]};
   const V3 d{a.dx[s], a.dy[s], a.dz[s]};
   double step;
-  FindNextBoundaryAndStep<D, false, M>(g, p, d, cur, nst, pstep, step, kind);
+  FindNextBoundaryAndStep<D, false, M, MX>(g, p, d, cur, nst, pstep, step, kind);
   const double sp = step + 1E-6;
   p = V3{p.x + d.x * sp, p.y + d.y * sp, p.z + d.z * sp};
   a.x[s] = p.x;
@@ -232,8 +232,9 @@ void StartLaunch(vg_server *s, int k) {
     rc = CheckHip(hipMemcpyAsync(s->d_segs[k], s->h_segs[k], nseg * sizeof(Seg), hipMemcpyHostToDevice, s->stream[k]),
                   "serve segment upload");
     if (!rc) {
-      VGA_NAV_VARIANTS(s->g, ServeStepKernel, dim3((unsigned)tiles), dim3(kServeBlock), 0, s->stream[k], s->dev, s->a,
-                       s->d_segs[k], nseg, tiles, s->pstep, s->navigator);
+      VGA_NAV_VARIANTS_MX(s->g, s->navigator == VG_NAV_ABBOX_MIXED, ServeStepKernel, dim3((unsigned)tiles),
+                          dim3(kServeBlock), 0, s->stream[k], s->dev, s->a, s->d_segs[k], nseg, tiles, s->pstep,
+                          s->navigator);
       rc = CheckHip(hipGetLastError(), "serve step launch");
     }
   }
@@ -436,7 +437,7 @@ int vg_server_create(const vg_geometry *g, uint64_t hbm_budget_bytes, int32_t na
   s->g = g;
   s->navigator = navigator;
   s->pstep = pstep;
-  s->dev = NavDev{g->d_placed, g->d_params, g->d_daughters, g->d_aabb, g->d_b32, g->d_b32side,
+  s->dev = NavDev{g->d_placed, g->d_params, g->d_daughters, g->d_aabb, g->d_b32, g->d_b32side, g->d_slot_aabb,
                   g->world,    (int32_t)g->placed.size(), (int32_t)g->daughters.size(), 0};
   rc = CheckHip(hipMalloc(&s->mem, cap * SlotBytes(depth)), "hipMalloc(serve arena)");
   if (!rc) {
