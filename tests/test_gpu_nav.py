@@ -22,6 +22,10 @@ from vecgeom_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 KINDS = ((N.ABBOX, "abbox"), (N.SIMPLE, "simple"))
+# The device navigators and the oracle navigator each must equal: the
+# mixed-precision ABBox navigator gives the ABBox navigator's results.
+DEVICE_KINDS = KINDS + ((N.ABBOX_MIXED, "abbox"),)
+ORACLE_KIND = {"abbox": N.ABBOX, "simple": N.SIMPLE}
 ZOO_REF_NAMES = [n for n, _ in W.NAV_ZOO if n != "Parallelepiped"]
 
 
@@ -53,7 +57,7 @@ def check_nav(geom, nav_ref, g, device):
     assert (lvl == g["level"]).all()
     assert (path == g["path"]).all()
     assert (bnd == 0).all()
-    for kind, tag in KINDS:
+    for kind, tag in DEVICE_KINDS:
         new, step = N.find_next_boundary_and_step(geom, p, d, s, navigator=kind)
         lo, pa, bo = states_host(new)
         assert (lo == g[f"step_{tag}_level"]).all(), tag
@@ -123,8 +127,8 @@ def test_gpu_nav_limited_pstep_vs_oracle(gpu, oracle, nested_dev):
     p = p + d * 150.0  # start between the inner shells
     nav = po.Nav(oracle, flat)
     level, path = nav.locate(p)
-    for kind, tag in KINDS:
-        lo, pa, bo, st = nav.step(p, d, level, path, pstep=7.5, kind=kind)
+    for kind, tag in DEVICE_KINDS:
+        lo, pa, bo, st = nav.step(p, d, level, path, pstep=7.5, kind=ORACLE_KIND[tag])
         pd, dd = dev(p, gpu), dev(d, gpu)
         s = N.locate_points(geom, pd)
         ps = torch.full((p.shape[1],), 7.5, dtype=torch.float64, device=gpu)
@@ -182,6 +186,14 @@ def test_gpu_nav_full_size_config(gpu, oracle, nested_dev):
     assert close(fin[:, idx], ofp, 1 + ons).all()
     exact = host(t.exact)
     print(f"steps {int(ns.sum())}, mean {ns.mean():.2f}, exact-path steps {int(exact.sum())}")
+    # The mixed-precision navigator: every one of the 2^20 tracks identical
+    # to the FP64 navigator's, bit for bit.
+    m = N.trace_tracks(geom, dev(p0, gpu), dev(d0, gpu), max_steps=100000, navigator=N.ABBOX_MIXED)
+    assert (host(m.nsteps) == ns).all()
+    assert (m.hash_u64() == t.hash_u64()).all()
+    for i in range(3):
+        assert np.array_equal(host(m.final[i]), fin[i])
+    assert np.array_equal(host(m.travelled), host(t.travelled))
 
 
 def test_gpu_nav_states_pool_c_abi(gpu, nested_dev):

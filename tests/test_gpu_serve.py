@@ -66,13 +66,15 @@ def test_server_basket_alone_equals_trace(gpu, nested):
     srv.destroy()
 
 
-def test_server_mixed_concurrent_clients_equal_alone(gpu, nested):
+@pytest.mark.parametrize("navigator", [N.ABBOX, N.ABBOX_MIXED])
+def test_server_mixed_concurrent_clients_equal_alone(gpu, nested, navigator):
     """Uneven baskets (1 .. 70000 tracks, and an empty one) stepped by four
-    client threads at once: every basket equals its own trace, and the server
-    merged requests into fewer launches than requests."""
+    client threads at once: every basket equals its own trace (the FP64 ABBox
+    navigator's, also for the mixed-precision server), and the server merged
+    requests into fewer launches than requests."""
     sizes = [1, 17, 128, 129, 4099, 70000, 0, 333, 2500, 9000, 31, 64]
     data = baskets(sizes, seed=900)
-    srv = S.Server(nested, 256 << 20)
+    srv = S.Server(nested, 256 << 20, navigator=navigator)
     ids = [srv.open(p, d) for p, d in data]
     errors = []
 
