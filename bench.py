@@ -42,6 +42,15 @@ PROFILE = os.path.join(ROOT, "profiles", "shape_zoo_fp64_kernels.md")
 BATCH_PROFILE = os.path.join(ROOT, "profiles", "shape_zoo_batched_kernels.md")
 
 
+# Weight classes of the batched launch (vg_batch.hip ClassOf; the
+# variable-length shapes' blocks of this workload fit in LDS: class 3).
+SHAPE_CLASS = {"Cone": 1, "Sphere": 1, "Torus": 2, "Polycone": 3, "Polyhedron": 3}
+
+
+def shape_class(name):
+    return SHAPE_CLASS.get(name, 0)
+
+
 def ClassIT(c, it):
     """Items per lane of weight class c for a batch launched with `it`."""
     return it if c == 0 else min(it, 2)
@@ -104,6 +113,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-FP64-form denominator")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched and FP32-storage forms")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision workload and forms")
     ap.add_argument("--nav-tracks", type=int, default=1 << 20, help="tracks of the nested-geometry navigation run")
     ap.add_argument("--nav-steps", type=int, default=3, help="timed trace launches")
     ap.add_argument("--no-nav", action="store_true")
@@ -167,6 +177,7 @@ class Workload:
                     del it["count"][q]
             self.items.append(it)
         self.names = [it["name"] for it in self.items]
+        self.names_of_pid = {it["pid"]: it["name"] for it in self.items}
         self.queries_per_step = sum(sum(it["count"].values()) for it in self.items)
         self.bytes_per_step = sum(c * BYTES[q] for it in self.items for q, c in it["count"].items())
         self.bytes_per_step_f32 = sum(c * BYTES_F32[q] for it in self.items for q, c in it["count"].items())
@@ -213,6 +224,89 @@ class Workload:
                 ((it["name"], "safe_out"), lambda it=it: Q.safety_to_out(g, it["pid"], it["pin"],
                                                                          out=it["out"]["safe_out"])),
             ]
+        return out
+
+
+class MixedWorkload:
+    """The mixed-precision workload of the zoo: the same shapes and
+    placements, 2^22 float32 points per shape of which a quarter lie on the
+    surface, jittered by N(0, (1e-7 mm)^2) (interior points moved along
+    isotropic directions by the FP64 DistanceToOut, as
+    workloads.surface_sampler describes), the rest uniform in 1.5x the
+    extent, with isotropic float32 directions.  The step is the FP64 step's
+    set of queries; inside / outside routing by the FP64 Contains of the
+    float points promoted to double."""
+
+    def __init__(self, wl, device):
+        import torch
+
+        from vecgeom_amd import workloads as W
+
+        Q = wl.Q
+        self.wl = wl
+        self.items = []
+        t64 = lambda a: tuple(torch.from_numpy(np.ascontiguousarray(a[i])).to(device) for i in range(3))
+        for k, ((name, u), xf, pid) in enumerate(zip(wl.shapes, wl.xfs, wl.ids)):
+            ext = wl.geom.placed_extent(pid)
+
+            def contains(lp, xf=xf, pid=pid):
+                m = W.local_to_master(xf, lp)
+                return Q.contains(wl.geom, pid, t64(m)).bool().cpu().numpy()
+
+            def dist_out(lp, ld, pid=pid):
+                return Q.distance_to_out(wl.geom, pid, t64(lp), t64(ld)).cpu().numpy()
+
+            inp = W.mixed_inputs(ext, xf, wl.n, W.SEED_BASE + 3 + 17 * k, W.surface_sampler(ext, contains, dist_out))
+            p32, d32 = inp["p32"], inp["d32"]
+            lp32, ld32 = inp["lp"].astype(np.float32), inp["ld"].astype(np.float32)
+            f = lambda a: tuple(torch.from_numpy(np.ascontiguousarray(a[i])).to(device) for i in range(3))
+            P, D, LP, LD = f(p32), f(d32), f(lp32), f(ld32)
+            c = Q.contains(wl.geom, pid, tuple(a.double() for a in P)).bool()
+            iin, iout = torch.nonzero(c).flatten(), torch.nonzero(~c).flatten()
+            pick = lambda s_, idx: tuple(a[idx].contiguous() for a in s_)
+            it = {"name": name, "pid": pid, "p": P, "pout": pick(P, iout), "dout": pick(D, iout),
+                  "pin": pick(LP, iin), "din": pick(LD, iin),
+                  "smout": torch.full((iout.numel(),), 1e30, dtype=torch.float64, device=device),
+                  "smin": torch.full((iin.numel(),), 1e30, dtype=torch.float64, device=device),
+                  "host": {"p": inp["p"], "d": inp["d"], "lp": lp32.astype(np.float64), "ld": ld32.astype(np.float64)},
+                  "inside_idx": iin.cpu().numpy(), "outside_idx": iout.cpu().numpy()}
+            n, n_in, n_out = wl.n, iin.numel(), iout.numel()
+            it["count"] = {"contains": n, "inside": n, "dist_in": n_out, "safe_in": n_out, "dist_out": n_in,
+                           "safe_out": n_in}
+            for q in list(it["count"]):
+                if (name, q) in EXCLUDED:
+                    del it["count"][q]
+            self.items.append(it)
+        self.queries_per_step = sum(sum(it["count"].values()) for it in self.items)
+        self.bytes_per_step = sum(c * BYTES_F32[q] for it in self.items for q, c in it["count"].items())
+        self.bytes_per_step_f64 = sum(c * BYTES[q] for it in self.items for q, c in it["count"].items())
+
+    def tasks(self, form, counts=None):
+        """The step as a task table: form "mixed" (float32 inputs, FP32
+        filter), "fp32" (float32 inputs promoted), or "fp64" (the float32
+        values promoted to float64 arrays).  Fresh output tensors; counts:
+        {(shape, query): one-element int64 tensor} for FP64-lane counts."""
+        import torch
+
+        from vecgeom_amd import batch as B
+
+        cv = (lambda s_: tuple(a.double() for a in s_)) if form == "fp64" else (lambda s_: s_)
+        mixed = form == "mixed"
+        out = {}
+        for it in self.items:
+            pid = it["pid"]
+            p, pout, dout, pin, din = cv(it["p"]), cv(it["pout"]), cv(it["dout"]), cv(it["pin"]), cv(it["din"])
+            c = (lambda q: counts.get((it["name"], q))) if counts is not None else (lambda q: None)
+            mk = {"contains": lambda: B.Task(pid, B.CONTAINS, p, mixed=mixed, fp64_count=c("contains")),
+                  "inside": lambda: B.Task(pid, B.INSIDE, p, mixed=mixed, fp64_count=c("inside")),
+                  "dist_in": lambda: B.Task(pid, B.DISTANCE_TO_IN, pout, dout, it["smout"], mixed=mixed,
+                                            fp64_count=c("dist_in")),
+                  "safe_in": lambda: B.Task(pid, B.SAFETY_TO_IN, pout, mixed=mixed, fp64_count=c("safe_in")),
+                  "dist_out": lambda: B.Task(pid, B.DISTANCE_TO_OUT, pin, din, it["smin"], mixed=mixed,
+                                             fp64_count=c("dist_out")),
+                  "safe_out": lambda: B.Task(pid, B.SAFETY_TO_OUT, pin, mixed=mixed, fp64_count=c("safe_out"))}
+            for q in it["count"]:
+                out[(it["name"], q)] = mk[q]()
         return out
 
 
@@ -342,14 +436,22 @@ def cpu_baseline(wl, target_s):
     }
 
 
-def sample_check(wl, promote, stride=61):
-    """Compares the step's current outputs with the oracle on every
-    stride-th query of each (shape, query), the oracle evaluated on the
-    inputs the device saw (float32 values promoted to double when promote).
-    Returns per shape: classification mismatches, distance/safety lanes
-    beyond max(1e-9 mm, 1e-12 relative), the largest |difference| and the
-    lanes compared."""
-    import torch
+SAMPLE_STRIDE = 61
+
+
+def capture(items, outputs, stride=SAMPLE_STRIDE):
+    """Every stride-th output of each (shape, query) of a form, as numpy:
+    classifications over all points, distances and safeties over their
+    basket.  outputs: {(shape, query): tensor}."""
+    return {k: v.cpu().numpy()[::stride].copy() for k, v in outputs.items()}
+
+
+def sample_check(items, shapes, xfs, n, captured, promote, stride=SAMPLE_STRIDE):
+    """Part of the CPU-baseline leg: compares sampled device outputs
+    (capture) with the oracle evaluated on the inputs the device saw (float32
+    values promoted to double when promote).  Returns per shape:
+    classification mismatches, distance/safety lanes beyond max(1e-9 mm,
+    1e-12 relative), the largest |difference| and the lanes compared."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
@@ -357,29 +459,26 @@ def sample_check(wl, promote, stride=61):
     cv = (lambda a: a.astype(np.float32).astype(np.float64)) if promote else (lambda a: a)
     qcode = {"contains": po.Q_CONTAINS, "inside": po.Q_INSIDE, "dist_in": po.Q_DIST_IN, "safe_in": po.Q_SAFE_IN,
              "dist_out": po.Q_DIST_OUT, "safe_out": po.Q_SAFE_OUT}
-    torch.cuda.synchronize()
     out = {}
-    for it, (name, u), xf in zip(wl.items, wl.shapes, wl.xfs):
+    for it, (name, u), xf in zip(items, shapes, xfs):
         h = o.placed(u.type, u.params, xf.array())
         hp = it["host"]
         r = {"class_mismatch": 0, "beyond_bar": 0, "max_abs_err": 0.0, "lanes": 0}
         for q in it["count"]:
-            dev = it["out"][q].cpu().numpy()
+            got = captured[(name, q)]
             if q in ("contains", "inside"):
-                idx = np.arange(0, wl.n, stride)
+                idx = np.arange(0, n, stride)
                 want = h.query(qcode[q], cv(hp["p"][:, idx]))
-                r["class_mismatch"] += int((dev[idx].astype(np.int64) != want.astype(np.int64)).sum())
+                r["class_mismatch"] += int((got.astype(np.int64) != want.astype(np.int64)).sum())
             else:
                 src_idx = it["outside_idx"] if q in ("dist_in", "safe_in") else it["inside_idx"]
-                k = np.arange(0, len(src_idx), stride)
-                sel = src_idx[k]
+                sel = src_idx[::stride]
                 P = cv(hp["p"][:, sel]) if q in ("dist_in", "safe_in") else cv(hp["lp"][:, sel])
                 if q in ("dist_in", "dist_out"):
                     D = cv(hp["d"][:, sel]) if q == "dist_in" else cv(hp["ld"][:, sel])
                     want = h.query(qcode[q], P, D, np.full(len(sel), 1e30))
                 else:
                     want = h.query(qcode[q], P)
-                got = dev[k]
                 eq = (got == want) | (np.isnan(got) & np.isnan(want))
                 with np.errstate(invalid="ignore"):
                     diff = np.where(eq, 0.0, np.abs(got - want))
@@ -390,7 +489,7 @@ def sample_check(wl, promote, stride=61):
                     r["max_abs_err"] = max(r["max_abs_err"], float(diff[fin].max()))
                 if (~fin & ~eq).any():
                     r["max_abs_err"] = float("inf")
-            r["lanes"] += len(dev[::stride]) if q in ("contains", "inside") else len(np.arange(0, len(dev), stride))
+            r["lanes"] += got.size
         out[name] = r
     return out
 
@@ -419,17 +518,47 @@ def batch_forms(args, wl, stream):
     for c in classes:
         c["achieved_gbs"] = round(c["bytes"] / (c["mean_ms"] * 1e-3) / 1e9, 1) if c["mean_ms"] > 0 else None
         c["mean_us"] = round(1e3 * c.pop("mean_ms"), 3)
-    res["classes"] = classes
-    dom = max(classes, key=lambda c: c["mean_us"])
+    res["classes_concurrent"] = classes
+    # The class kernels of the step run concurrently, so the events around
+    # each measure its span inside the step, not its own speed.  Each class
+    # is therefore also run alone (a batch of its tasks only, the same
+    # launch shape), timed with HIP events on its launch stream over
+    # args.steps runs; the class that takes longest alone is the dominant
+    # kernel of the step.
+    alone = []
+    for c in classes:
+        sub = [t for t in wl.tasks() if shape_class(wl.names_of_pid[t.placed]) == c["class"]]
+        if not sub:
+            continue
+        cb = B.Batch(wl.geom, sub)
+        cb.set_launch(it, blocks)
+        cb.set_timing(True)
+        for _ in range(max(1, args.warmup)):
+            cb.run()
+        cb.set_timing(True)
+        time_steps([(("class", c["class"]), cb.run)], args.steps, stream, watch=[])
+        st = cb.class_stats()[c["class"]]
+        cb.free()
+        ms = st["mean_ms"]
+        alone.append({"class": c["class"], "name": c["name"], "items": st["items"], "bytes": st["bytes"],
+                      "runs": st["runs"], "mean_us": round(1e3 * ms, 3),
+                      "achieved_gbs": round(st["bytes"] / (ms * 1e-3) / 1e9, 1) if ms > 0 else None})
+    res["classes_alone"] = alone
+    dom = max(alone, key=lambda c: c["mean_us"])
     res["roofline"] = {"bound": "hbm", "achieved": dom["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": round(dom["achieved_gbs"] / HBM_PEAK_GBS, 4),
                        "traffic": profiled_class_traffic(dom["class"], dom["items"]),
                        "kernel": f"BatchKernel<{ClassIT(dom['class'], it)},{dom['class']}> ({dom['name']})",
                        "avg_launch_us": dom["mean_us"], "queries_per_launch": dom["items"],
-                       "bytes_per_launch": dom["bytes"],
+                       "bytes_per_launch": dom["bytes"], "timed": "alone (its class's tasks only), HIP events on its "
+                                                                  "launch stream, mean of the timed runs",
                        "bytes_rule": "each input element read once, each output element written once (FP64: "
                                      "Contains 25 B, Inside 28, distances 64, safeties 32 per query)"}
-    res["check_vs_oracle"] = sample_check(wl, promote=False)
+    res["step_roofline"] = {"bound": "hbm", "achieved": res["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(res["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                            "what": "the whole batched step (all class kernels, concurrent): its algorithmic bytes "
+                                    "over its time per step"}
+    res["_captured"] = capture(wl.items, {(it["name"], q): it["out"][q] for it in wl.items for q in it["count"]})
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
@@ -456,9 +585,120 @@ def batch_forms(args, wl, stream):
             "achieved_gbs": round(wl.bytes_per_step_f32 / (el32 / args.steps) / 1e9, 1),
             "note": "float32 inputs promoted to double before any arithmetic; check_vs_oracle compares a sample "
                     "of the outputs with the oracle evaluated on the same promoted inputs"}
-    fp32["check_vs_oracle"] = sample_check(wl, promote=True)
+    fp32["_captured"] = capture(wl.items, {(it["name"], q): it["out"][q] for it in wl.items for q in it["count"]})
     f32.free()
     return res, fp32
+
+
+def mixed_forms(args, mwl, stream):
+    """The mixed-precision workload in four forms, each autotuned and timed
+    over the same number of steps: the mixed form (FP32 filter, FP64 on the
+    undecided lanes), FP32 storage (FP64 arithmetic on the promoted float
+    inputs), the FP64 batched form and the plain FP64 form on float64 copies
+    of the same values.  Per shape: mismatches and max / RMS |difference| of
+    the mixed outputs against the FP64 batched outputs (every lane), and the
+    fraction of lanes the mixed form evaluated in FP64 (one counted run)."""
+    import torch
+
+    from vecgeom_amd import batch as B
+    from vecgeom_amd import capi
+
+    res = {"workload": "shape zoo, random placements, 2^%d float32 points per shape, %d%% on the surface jittered "
+                       "by N(0, (1e-7 mm)^2), the rest uniform in 1.5x the extent" % (
+                           int(np.log2(mwl.wl.n)), 25),
+           "queries_per_step": mwl.queries_per_step}
+    outs = {}
+    for form in ("mixed", "mixed_all", "fp32", "fp64"):
+        # mixed: the FP32 filter where it was measured to pay; mixed_all: on
+        # every Contains, Inside and DistanceToIn task.
+        B.set_mixed_policy(B.POLICY_ALL if form == "mixed_all" else B.POLICY_MEASURED)
+        tasks = mwl.tasks("mixed" if form.startswith("mixed") else form)
+        bt = B.Batch(mwl.wl.geom, list(tasks.values()))
+        bt.autotune(3)
+        it, blocks, _ = bt.launch()
+        for _ in range(max(1, args.warmup)):
+            bt.run()
+        el, _ = time_steps([(("batch", form), bt.run)], args.steps, stream, watch=[])
+        nbytes = mwl.bytes_per_step if form != "fp64" else mwl.bytes_per_step_f64
+        res[form] = {"value": round(mwl.queries_per_step * args.steps / el / 1e9, 3), "unit": "Gqueries/s",
+                     "ms_per_step": round(1e3 * el / args.steps, 4), "items_per_lane": it, "blocks": blocks,
+                     "achieved_gbs": round(nbytes / (el / args.steps) / 1e9, 1)}
+        torch.cuda.synchronize()
+        outs[form] = {k: t.out for k, t in tasks.items()}
+        bt.free()
+    B.set_mixed_policy(B.POLICY_MEASURED)
+    # The plain FP64 form on the float64 copies: one launch per (shape, query).
+    Q, g = mwl.wl.Q, mwl.wl.geom
+    plain = []
+    for it in mwl.items:
+        d64 = lambda s_: tuple(a.double() for a in s_)
+        P, PO, DO, PI, DI = d64(it["p"]), d64(it["pout"]), d64(it["dout"]), d64(it["pin"]), d64(it["din"])
+        fns = {"contains": lambda P=P, pid=it["pid"]: Q.contains(g, pid, P),
+               "inside": lambda P=P, pid=it["pid"]: Q.inside(g, pid, P),
+               "dist_in": lambda PO=PO, DO=DO, it=it: Q.distance_to_in(g, it["pid"], PO, DO, it["smout"]),
+               "safe_in": lambda PO=PO, it=it: Q.safety_to_in(g, it["pid"], PO),
+               "dist_out": lambda PI=PI, DI=DI, it=it: Q.distance_to_out(g, it["pid"], PI, DI, it["smin"]),
+               "safe_out": lambda PI=PI, it=it: Q.safety_to_out(g, it["pid"], PI)}
+        plain += [((it["name"], q), fns[q]) for q in it["count"]]
+    capi.call("vg_set_kernel_variant", 1)
+    for _ in range(max(1, args.warmup)):
+        for _, fn in plain:
+            fn()
+    pel, _ = time_steps(plain, args.steps, stream, watch=[])
+    capi.call("vg_set_kernel_variant", 0)
+    res["plain_fp64_gpu"] = {"value": round(mwl.queries_per_step * args.steps / pel / 1e9, 3), "unit": "Gqueries/s",
+                             "ms_per_step": round(1e3 * pel / args.steps, 4)}
+    res["speedup_vs_plain_fp64"] = round(res["mixed"]["value"] / res["plain_fp64_gpu"]["value"], 3)
+    res["speedup_vs_fp64_batched"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
+    res["policy"] = ("mixed: FP32 filter on the (shape, query) pairs where it was measured to pay (Polycone "
+                     "DistanceToIn), FP32 storage elsewhere; mixed_all: FP32 filter on every Contains, Inside and "
+                     "DistanceToIn task.  On gfx950 an FP64 FMA issues at the unpacked FP32 rate.")
+    # Errors of the mixed forms against the FP64 form on the same values
+    # (every lane).
+    for form in ("mixed", "mixed_all"):
+        err = {}
+        for it in mwl.items:
+            r = {"class_mismatch": 0, "max_abs_err": 0.0, "rms_err": 0.0, "lanes": 0}
+            sq, nd = 0.0, 0
+            for q in it["count"]:
+                a = outs[form][(it["name"], q)].cpu().numpy()
+                b = outs["fp64"][(it["name"], q)].cpu().numpy()
+                r["lanes"] += a.size
+                if q in ("contains", "inside"):
+                    r["class_mismatch"] += int((a != b).sum())
+                    continue
+                eq = (a == b) | (np.isnan(a) & np.isnan(b))
+                with np.errstate(invalid="ignore"):
+                    dif = np.where(eq, 0.0, np.abs(a - b))
+                if not np.isfinite(dif).all():
+                    r["max_abs_err"] = float("inf")
+                else:
+                    r["max_abs_err"] = max(r["max_abs_err"], float(dif.max(initial=0.0)))
+                    sq += float((dif * dif).sum())
+                    nd += dif.size
+            r["rms_err"] = (sq / nd) ** 0.5 if nd else 0.0
+            err[it["name"]] = r
+        res[form]["error_vs_fp64"] = err
+    for form, policy in (("mixed", B.POLICY_MEASURED), ("mixed_all", B.POLICY_ALL)):
+        B.set_mixed_policy(policy)
+        counts = {(it["name"], q): torch.zeros(1, dtype=torch.int64, device=mwl.items[0]["p"][0].device)
+                  for it in mwl.items for q in it["count"]}
+        tasks = mwl.tasks("mixed", counts)
+        bt = B.Batch(mwl.wl.geom, list(tasks.values()))
+        bt.run()
+        torch.cuda.synchronize()
+        bt.free()
+        frac = {}
+        tot64 = 0
+        for it in mwl.items:
+            frac[it["name"]] = {q: round(int(counts[(it["name"], q)].item()) / max(1, c), 4)
+                                for q, c in it["count"].items()}
+            tot64 += sum(int(counts[(it["name"], q)].item()) for q in it["count"])
+        res[form]["fp64_lane_fraction"] = frac
+        res[form]["fp64_lane_fraction_all"] = round(tot64 / mwl.queries_per_step, 4)
+    B.set_mixed_policy(B.POLICY_MEASURED)
+    res["_captured"] = capture(mwl.items, outs["mixed"])
+    return res
 
 
 def nav_bench(args, dev, stream, with_cpu):
@@ -502,6 +742,27 @@ def nav_bench(args, dev, stream, with_cpu):
            "exact_walk_steps": exact, "kernel": "TraceKernel (fused TraceTrack loop)",
            "avg_launch_ms": round(fused_ms, 3), "value": round(total_steps / (fused_ms * 1e-3) / 1e9, 4),
            "unit": "G steps/s", "dtype": "f64"}
+    # The mixed-precision navigator (FP32 filters in the daughter walk and
+    # the locate scans), same tracks; identical results required.
+    ref_ns, ref_hash = out.nsteps.clone(), out.hash.clone()
+    ref_fin = [a.clone() for a in out.final]
+    mout = N.TraceResult(n, dev)
+    N.trace_tracks(geom, p, d, navigator=N.ABBOX_MIXED, out=mout)
+    torch.cuda.synchronize()
+    ev = []
+    for _ in range(args.nav_steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        N.trace_tracks(geom, p, d, navigator=N.ABBOX_MIXED, out=mout)
+        e1.record(stream)
+        ev.append((e0, e1))
+    torch.cuda.synchronize()
+    mixed_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    same = bool(torch.equal(mout.nsteps, ref_ns) and torch.equal(mout.hash, ref_hash)
+                and all(torch.equal(a, b) for a, b in zip(mout.final, ref_fin)))
+    res["mixed"] = {"kernel": "TraceKernel<mixed> (FP32 daughter and locate filters)", "avg_launch_ms": round(mixed_ms, 3),
+                    "value": round(total_steps / (mixed_ms * 1e-3) / 1e9, 4), "unit": "G steps/s",
+                    "identical_to_fp64": same, "speedup_vs_fused_fp64": round(fused_ms / mixed_ms, 3)}
     # Plain form: one launch per step from the host.
     cur = N.locate_points(geom, p)
     new = N.NavStates(geom, n, device=dev)
@@ -525,6 +786,7 @@ def nav_bench(args, dev, stream, with_cpu):
     res["plain_per_step_launch"] = {"value": round(total_steps / pel / 1e9, 4), "unit": "G steps/s",
                                     "launches": launches, "seconds": round(pel, 4),
                                     "speedup_of_fused": round((total_steps / (fused_ms * 1e-3)) / (total_steps / pel), 3)}
+    res["mixed"]["speedup_vs_plain_fp64"] = round((total_steps / (mixed_ms * 1e-3)) / (total_steps / pel), 3)
     if with_cpu:
         res["cpu_baseline"] = nav_cpu_baseline(flat, p0, d0, args.cpu_seconds / 2)
     geom.free()
@@ -537,11 +799,12 @@ def serve_bench(args, dev):
     arrive as a seeded Poisson process, are admitted by footprint into a
     32 GiB track arena, take `serve_steps` navigation steps each (one request
     per step, the next submitted when the previous completes) and close.
-    Latency is per request, submission to completion.  The batched form
-    merges all pending requests into one launch on each of two streams; the
-    per-basket form launches each request alone, one at a time.  Rates are
-    0.25, 0.5 and 0.9 of the per-basket form's saturation rate and 0.5 and
-    0.9 of the batched form's (saturation: all baskets arriving at once)."""
+    Latency is per request, submission to completion.  The batched forms
+    merge all pending requests into one launch on each of two streams, with
+    the mixed-precision or the FP64 navigator; the per-basket form launches
+    each request alone, one at a time, with the FP64 navigator.  Rates are
+    0.5 and 0.9 of the per-basket form's saturation rate and 0.5 and 0.9 of
+    the batched form's (saturation: all baskets arriving at once)."""
     from vecgeom_amd import geometry as G
     from vecgeom_amd import serve as S
     from vecgeom_amd import workloads as W
@@ -552,11 +815,15 @@ def serve_bench(args, dev):
     nb, steps = args.serve_baskets, args.serve_steps
     sizes = np.exp(rng.uniform(np.log(16), np.log(65536), nb)).astype(np.uint64)
     p, d = W.track_starts(int(sizes.sum()), seed=4040)
-    srv = S.Server(geom, 32 << 30)
-    forms = {"batched": (4096, 2), "per_basket": (1, 1)}
+    from vecgeom_amd import nav as N
+
+    servers = {"fp64": S.Server(geom, 32 << 30), "mixed": S.Server(geom, 32 << 30, navigator=N.ABBOX_MIXED)}
+    # form: (server, max requests per launch, launch streams)
+    forms = {"batched_mixed": ("mixed", 4096, 2), "batched": ("fp64", 4096, 2), "per_basket": ("fp64", 1, 1)}
 
     def run(form, arrival):
-        srv.set_batching(*forms[form])
+        srv = servers[forms[form][0]]
+        srv.set_batching(*forms[form][1:])
         s0 = srv.stats()
         lat, wait, wall = srv.replay(arrival, sizes, steps, p, d)
         s1 = srv.stats()
@@ -569,12 +836,13 @@ def serve_bench(args, dev):
                 "requests": req, "launches": s1["launches"] - s0["launches"]}
 
     zero = np.zeros(nb)
-    run("batched", zero)  # warm-up
+    run("batched", zero)  # warm-ups
+    run("batched_mixed", zero)
     sat = {f: run(f, zero) for f in forms}
     base_rate = nb / sat["per_basket"]["wall_s"]
     batched_rate = nb / sat["batched"]["wall_s"]
     rates = {}
-    for rate in [f * base_rate for f in (0.25, 0.5, 0.9)] + [f * batched_rate for f in (0.5, 0.9)]:
+    for rate in [f * base_rate for f in (0.5, 0.9)] + [f * batched_rate for f in (0.5, 0.9)]:
         gaps = rng.exponential(1.0 / rate, nb)
         arrival = np.cumsum(gaps) - gaps[0]
         rates[f"{rate:.0f}/s"] = {f: run(f, arrival) for f in forms}
@@ -583,8 +851,11 @@ def serve_bench(args, dev):
            "unit_steps": "G steps/s", "saturation": sat,
            "saturation_basket_rate_per_basket_form": round(base_rate, 1),
            "saturation_basket_rate_batched_form": round(batched_rate, 1),
-           "fixed_rates": rates, "dtype": "f64"}
-    srv.destroy()
+           "fixed_rates": rates, "dtype": "f64",
+           "forms": "batched_mixed: requests merged into shared launches, mixed-precision ABBox navigator; batched: "
+                    "the same with the FP64 navigator; per_basket: one launch per request, one at a time, FP64"}
+    for srv in servers.values():
+        srv.destroy()
     geom.free()
     return res
 
@@ -719,6 +990,12 @@ def main():
     if not args.no_batch:
         log("batched forms")
         batched, fp32 = batch_forms(args, wl, stream)
+    mixed = mwl = None
+    if not args.no_mixed:
+        log("mixed-precision workload")
+        mwl = MixedWorkload(wl, dev)
+        log("mixed-precision forms")
+        mixed = mixed_forms(args, mwl, stream)
     forms = {"per_launch": value}
     if batched:
         forms["batched"] = batched["value"]
@@ -737,6 +1014,17 @@ def main():
     if not args.no_cpu:
         log("cpu baseline (shapes)")
         cpu = cpu_baseline(wl, args.cpu_seconds)
+        # The CPU leg also checks sampled device outputs against the oracle
+        # on the inputs the device saw.
+        log("sample checks against the oracle")
+        for form, promote in ((batched, False), (fp32, True)):
+            if form is not None:
+                form["check_vs_oracle"] = sample_check(wl.items, wl.shapes, wl.xfs, wl.n, form["_captured"], promote)
+        if mixed is not None:
+            mixed["check_vs_oracle"] = sample_check(mwl.items, wl.shapes, wl.xfs, wl.n, mixed["_captured"], False)
+    for form in (batched, fp32, mixed):
+        if form is not None:
+            form.pop("_captured", None)
 
     nav = None
     if not args.no_nav:
@@ -757,6 +1045,7 @@ def main():
         "per_launch_value": round(value_per_launch, 3),
         "batched": batched,
         "fp32_storage": fp32,
+        "mixed": mixed,
         "plain_fp64_gpu": plain,
         "cpu_baseline": cpu,
         "nav": nav,

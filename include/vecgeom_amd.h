@@ -230,7 +230,8 @@ int vg_safety_to_out_minimize(const vg_geometry *g, int32_t placed, const double
 /* Input element types: VG_F64 = double arrays (the SOA3D<double> of the
  * reference); VG_F32 = float arrays (FP32 storage: every value is promoted to
  * double before any arithmetic, so results equal the FP64 form on the same
- * float-valued inputs); VG_MIXED = float arrays with FP32 arithmetic first:
+ * float-valued inputs); VG_MIXED = float arrays with FP32 arithmetic first
+ * (subject to vg_set_mixed_policy below):
  * Contains and Inside are decided in FP32 wherever an FP32 lower bound on
  * the point's distance to the surface exceeds its rigorous rounding margin,
  * DistanceToIn returns kInfinity in FP32 for rays that provably miss the
@@ -241,6 +242,18 @@ int vg_safety_to_out_minimize(const vg_geometry *g, int32_t placed, const double
 #define VG_F64 0
 #define VG_F32 1
 #define VG_MIXED 2
+
+/* Which VG_MIXED tasks run the FP32 filter.  VG_MIXED_POLICY_MEASURED (the
+ * default): only the (shape, query) pairs where the filter was measured to
+ * save time on MI355X (an FP64 FMA issues at the rate of an unpacked FP32
+ * one on gfx950, so the filter pays only where it skips long FP64 work);
+ * the other VG_MIXED tasks run as VG_F32.  VG_MIXED_POLICY_ALL: every
+ * Contains, Inside and DistanceToIn task.  Results are the same under
+ * either policy; it applies to batches created and queries issued after
+ * the call.  No counterpart in the reference. */
+#define VG_MIXED_POLICY_MEASURED 0
+#define VG_MIXED_POLICY_ALL 1
+int vg_set_mixed_policy(int32_t policy);
 
 typedef struct vg_query_task {
   int32_t placed;     /* placed-volume id */
@@ -265,8 +278,9 @@ int vg_query(const vg_geometry *g, const vg_query_task *task, void *stream);
 
 /*
  * A batch: a fixed table of tasks (any mix of placed volumes, queries, basket
- * sizes and input types) run by one kernel launch per shape weight class (at
- * most five, issued on forked streams and joined back to the caller's), each
+ * sizes and input types) run by one kernel launch per shape weight class and
+ * precision (at most ten, issued on forked streams and joined back to the
+ * caller's), each
  * workgroup tile belonging to one task.  This replaces the per-shape, per-query launches of
  * Benchmarker::Run*Cuda (source/benchmarking/Benchmarker.cu:86-367).  The
  * arrays named in the tasks must stay valid while the batch is used; the
@@ -295,7 +309,8 @@ int vg_batch_autotune(vg_batch *b, int32_t repetitions, void *stream, float *bes
  * last 64 runs are kept).  Enabling clears earlier records. */
 int vg_batch_set_timing(vg_batch *b, int32_t enable);
 /* Weight class `cls` (0 light, 1 Cone/Sphere, 2 Torus, 3 Polycone/Polyhedron
- * staged in LDS, 4 the same read from global memory): its queries per run,
+ * staged in LDS, 4 the same read from global memory; 5-9 the same classes
+ * for VG_MIXED tasks, which run in kernels of their own): its queries per run,
  * its algorithmic bytes per run (each input element read once, each output
  * written once), and the number of timed runs and their mean kernel
  * duration (waits for the recorded events).  Each pointer is nullable. */

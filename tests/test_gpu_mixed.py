@@ -46,6 +46,15 @@ def same_lanes(a, b):
     return a == b
 
 
+@pytest.fixture(scope="module", autouse=True)
+def filter_everywhere(gpu):
+    """These tests exercise the FP32 filter on every shape: the policy that
+    runs it for every Contains, Inside and DistanceToIn task."""
+    B.set_mixed_policy(B.POLICY_ALL)
+    yield
+    B.set_mixed_policy(B.POLICY_MEASURED)
+
+
 @pytest.fixture(scope="module")
 def mixed_zoo(gpu, oracle):
     shapes = [(n, u) for n, u in W.zoo_shapes() if capi.lib().vg_shape_supported(u.type)]
@@ -144,3 +153,23 @@ def test_mixed_rejects_float64_inputs(gpu, mixed_zoo):
     p = tensors(data[0]["p"], gpu)
     with pytest.raises(ValueError, match="float32"):
         B.Task(ids[0], B.CONTAINS, p, mixed=True)
+
+
+def test_measured_policy_same_results(gpu, mixed_zoo):
+    """Under the measured policy (the default) only the filters that pay run
+    (Polycone DistanceToIn); every result is the same as under the
+    filter-everywhere policy, and the other tasks count every lane as FP64."""
+    shapes, xfs, geom, ids, data = mixed_zoo
+    try:
+        for k, inp in enumerate(data):
+            for q in QUERIES:
+                B.set_mixed_policy(B.POLICY_ALL)
+                a = B.query(geom, make_task(ids[k], q, inp, gpu, True)).cpu().numpy()
+                B.set_mixed_policy(B.POLICY_MEASURED)
+                c = torch.zeros(1, dtype=torch.int64, device=gpu)
+                b = B.query(geom, make_task(ids[k], q, inp, gpu, True, c)).cpu().numpy()
+                assert same_lanes(a, b).all(), (shapes[k][0], NAMES[q])
+                filtered = shapes[k][0] == "Polycone" and q == B.DISTANCE_TO_IN
+                assert (int(c.item()) < N) == filtered, (shapes[k][0], NAMES[q], int(c.item()))
+    finally:
+        B.set_mixed_policy(B.POLICY_ALL)

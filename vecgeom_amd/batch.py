@@ -18,6 +18,8 @@ from . import capi
 
 CONTAINS, INSIDE, DISTANCE_TO_IN, DISTANCE_TO_OUT, PLACED_DISTANCE_TO_OUT, SAFETY_TO_IN, SAFETY_TO_OUT = range(7)
 F64, F32, MIXED = 0, 1, 2
+# vg_set_mixed_policy: the FP32 filter where measured to pay, or everywhere.
+POLICY_MEASURED, POLICY_ALL = 0, 1
 OUT_DTYPE = {CONTAINS: torch.uint8, INSIDE: torch.int32}
 DIRECTIONAL = {DISTANCE_TO_IN, DISTANCE_TO_OUT, PLACED_DISTANCE_TO_OUT}
 
@@ -41,6 +43,7 @@ class CQueryTask(ctypes.Structure):
 
 _T = ctypes.POINTER(CQueryTask)
 capi.register({
+    "vg_set_mixed_policy": [ctypes.c_int32],
     "vg_query": [c_vp, _T, c_vp],
     "vg_batch_create": [c_vp, _T, ctypes.c_int32, ctypes.POINTER(c_vp)],
     "vg_batch_run": [c_vp, c_vp],
@@ -54,9 +57,12 @@ capi.register({
                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_float)],
 })
 
-# Weight classes of a batch (one kernel launch each).
-CLASS_NAMES = ["light (Box, Tube, Orb, Trd, Trapezoid, Parallelepiped, Paraboloid)", "Cone, Sphere", "Torus",
-               "Polycone, Polyhedron (parameters staged in LDS)", "Polycone, Polyhedron (parameters in global memory)"]
+# Weight classes of a batch (one kernel launch each); mixed-precision tasks
+# run in classes of their own, 5 + the base class.
+_BASE_CLASS_NAMES = ["light (Box, Tube, Orb, Trd, Trapezoid, Parallelepiped, Paraboloid)", "Cone, Sphere", "Torus",
+                     "Polycone, Polyhedron (parameters staged in LDS)",
+                     "Polycone, Polyhedron (parameters in global memory)"]
+CLASS_NAMES = _BASE_CLASS_NAMES + [f"mixed: {n}" for n in _BASE_CLASS_NAMES]
 
 
 def _stream():
@@ -107,6 +113,12 @@ class Task:
         return CQueryTask(int(self.placed), int(self.query), self.input_type, 0, self.n,
                           *[ptr(a) for a in self.p], *[ptr(a) for a in d], ptr(self.step_max), self.out.data_ptr(),
                           ptr(self.fp64_count))
+
+
+def set_mixed_policy(policy):
+    """Selects which mixed-precision tasks run the FP32 filter (applies to
+    batches and queries made afterwards; results do not depend on it)."""
+    capi.call("vg_set_mixed_policy", int(policy))
 
 
 def query(geom, task):

@@ -79,13 +79,28 @@ __device__ __forceinline__ void WaveCount(uint64_t *count, uint32_t n) {
   if (total && (uint32_t)__lane_id() == (uint32_t)(__ffsll((long long)act) - 1)) atomicAdd((unsigned long long *)count, total);
 }
 
-// FP32 filter of a mixed task for query Q at float point p (direction d):
-// true with the result in r when decided.
-template <int Q>
+// The shape kind of a shape kernel type (bit of the FP32 bound masks).
+template <class S> struct KindOf;
+template <> struct KindOf<BoxS> { static constexpr int v = VG_BOX; };
+template <> struct KindOf<TubeS> { static constexpr int v = VG_TUBE; };
+template <> struct KindOf<OrbS> { static constexpr int v = VG_ORB; };
+template <> struct KindOf<TrdS> { static constexpr int v = VG_TRD; };
+template <> struct KindOf<TrapS> { static constexpr int v = VG_TRAPEZOID; };
+template <> struct KindOf<ParaS> { static constexpr int v = VG_PARALLELEPIPED; };
+template <> struct KindOf<ParabS> { static constexpr int v = VG_PARABOLOID; };
+template <> struct KindOf<SphereS> { static constexpr int v = VG_SPHERE; };
+template <> struct KindOf<ConeS> { static constexpr int v = VG_CONE; };
+template <> struct KindOf<TorusS> { static constexpr int v = VG_TORUS; };
+template <> struct KindOf<PolyconeS> { static constexpr int v = VG_POLYCONE; };
+template <> struct KindOf<PolyhedronS> { static constexpr int v = VG_POLYHEDRON; };
+
+// FP32 filter of a mixed task of shape S for query Q at float point p
+// (direction d): true with the result in r when decided.
+template <class S, int Q>
 __device__ __forceinline__ bool Filter32(const TaskDev &t, F3 p, F3 d, double &r) {
   if constexpr (Q == kQDistIn) {
     if (!RayMissesBox32(*t.b32, t.ext, p, d)) return false;
-    if (t.type == VG_ORB) {  // the Orb's miss value depends on the ray's sense
+    if constexpr (KindOf<S>::v == VG_ORB) {  // the Orb's miss value depends on the ray's sense
       const int sg = RadialSign32(*t.b32, p, d);
       if (sg == 0) return false;
       r = sg < 0 ? __builtin_nan("") : kInfinity;
@@ -94,7 +109,7 @@ __device__ __forceinline__ bool Filter32(const TaskDev &t, F3 p, F3 d, double &r
     r = kInfinity;
     return true;
   } else {
-    const int c = Classify32(*t.b32, t.b32side, p);
+    const int c = Classify32<1u << KindOf<S>::v>(*t.b32, t.b32side, p);
     if (c < 0) return false;
     r = Q == kQContains ? (c == kInside ? 1. : 0.) : (double)c;
     return true;
@@ -128,8 +143,45 @@ __device__ __forceinline__ void LoadAll(const TaskDev t, uint64_t i0, bool full,
     Load<double, IT>(a, i0, full, vec, t.n, o);
 }
 
-// blk: the task's parameter block (in global memory or staged in LDS).
-template <class S, int Q, int IT>
+__device__ __forceinline__ double LoadOne(const TaskDev &t, const void *a, uint64_t i) {
+  return t.in_type != VG_F64 ? (double)static_cast<const float *>(a)[i] : static_cast<const double *>(a)[i];
+}
+
+// The FP64 evaluation of query Q for one item at promoted point p (and
+// direction d, step max sm); writes out[i].
+template <class S, int Q>
+__device__ __forceinline__ void Eval64(const TaskDev &t, const typename S::P &P, uint64_t i, V3 p, V3 d, double sm) {
+  if constexpr (Q == kQContains) {
+    static_cast<uint8_t *>(t.out)[i] = S::Contains(P, XfPointRt(t.xf, t.xfkind, p)) ? 1 : 0;
+  } else if constexpr (Q == kQInside) {
+    static_cast<int32_t *>(t.out)[i] = S::Inside(P, XfPointRt(t.xf, t.xfkind, p));
+  } else if constexpr (Q == kQDistIn) {
+    static_cast<double *>(t.out)[i] = S::DistanceToIn(P, XfPointRt(t.xf, t.xfkind, p), XfDirRt(t.xf, t.xfkind, d), sm);
+  } else if constexpr (Q == kQDistOut) {
+    static_cast<double *>(t.out)[i] = S::DistanceToOut(P, p, d, sm);
+  } else if constexpr (Q == kQPlacedDistOut) {
+    static_cast<double *>(t.out)[i] = S::DistanceToOut(P, XfPointRt(t.xf, t.xfkind, p), XfDirRt(t.xf, t.xfkind, d), sm);
+  } else if constexpr (Q == kQSafeIn) {
+    static_cast<double *>(t.out)[i] = S::SafetyToIn(P, XfPointRt(t.xf, t.xfkind, p));
+  } else {
+    static_cast<double *>(t.out)[i] = S::SafetyToOut(P, p);
+  }
+}
+
+// Writes a result decided by the FP32 filter.
+template <int Q>
+__device__ __forceinline__ void WriteDecided(const TaskDev &t, uint64_t i, double r) {
+  if constexpr (Q == kQContains)
+    static_cast<uint8_t *>(t.out)[i] = (uint8_t)r;
+  else if constexpr (Q == kQInside)
+    static_cast<int32_t *>(t.out)[i] = (int32_t)r;
+  else
+    static_cast<double *>(t.out)[i] = r;
+}
+
+// blk: the task's parameter block (in global memory or staged in LDS).  MX:
+// a mixed-precision task (FP32 filter first, per lane).
+template <class S, int Q, int IT, bool MX>
 __device__ __forceinline__ void RunTile(const TaskDev t, const double *blk, uint64_t i0) {
   const auto &P = ParamView<S>::Get(blk);
   const bool full = i0 + IT <= t.n;
@@ -145,8 +197,7 @@ __device__ __forceinline__ void RunTile(const TaskDev t, const double *blk, uint
 #pragma unroll
     for (int k = 0; k < IT; ++k) sm[k] = (t.sm && i0 + k < t.n) ? t.sm[i0 + k] : kInfinity;
   }
-  constexpr bool filt = (Q == kQContains || Q == kQInside || Q == kQDistIn);
-  const bool mixed = t.in_type == VG_MIXED;
+  constexpr bool filt = MX && (Q == kQContains || Q == kQInside || Q == kQDistIn);
   uint32_t n64 = 0;
 #pragma unroll
   for (int k = 0; k < IT; ++k) {
@@ -154,69 +205,119 @@ __device__ __forceinline__ void RunTile(const TaskDev t, const double *blk, uint
     const V3 p{px[k], py[k], pz[k]};
     const uint64_t i = i0 + k;
     if constexpr (filt) {
-      if (mixed) {
-        double r;
-        const F3 d32 = dir ? F3{(float)dx[k], (float)dy[k], (float)dz[k]} : F3{0.f, 0.f, 0.f};
-        if (Filter32<Q>(t, F3{(float)p.x, (float)p.y, (float)p.z}, d32, r)) {
-          if constexpr (Q == kQContains)
-            static_cast<uint8_t *>(t.out)[i] = (uint8_t)r;
-          else if constexpr (Q == kQInside)
-            static_cast<int32_t *>(t.out)[i] = (int32_t)r;
-          else
-            static_cast<double *>(t.out)[i] = r;
-          continue;
-        }
+      double r;
+      const F3 d32 = dir ? F3{(float)dx[k], (float)dy[k], (float)dz[k]} : F3{0.f, 0.f, 0.f};
+      if (Filter32<S, Q>(t, F3{(float)p.x, (float)p.y, (float)p.z}, d32, r)) {
+        WriteDecided<Q>(t, i, r);
+        continue;
       }
     }
     ++n64;
-    if constexpr (Q == kQContains) {
-      static_cast<uint8_t *>(t.out)[i] = S::Contains(P, XfPointRt(t.xf, t.xfkind, p)) ? 1 : 0;
-    } else if constexpr (Q == kQInside) {
-      static_cast<int32_t *>(t.out)[i] = S::Inside(P, XfPointRt(t.xf, t.xfkind, p));
-    } else if constexpr (Q == kQDistIn) {
-      const V3 d{dx[k], dy[k], dz[k]};
-      static_cast<double *>(t.out)[i] =
-          S::DistanceToIn(P, XfPointRt(t.xf, t.xfkind, p), XfDirRt(t.xf, t.xfkind, d), sm[k]);
-    } else if constexpr (Q == kQDistOut) {
-      static_cast<double *>(t.out)[i] = S::DistanceToOut(P, p, V3{dx[k], dy[k], dz[k]}, sm[k]);
-    } else if constexpr (Q == kQPlacedDistOut) {
-      const V3 d{dx[k], dy[k], dz[k]};
-      static_cast<double *>(t.out)[i] =
-          S::DistanceToOut(P, XfPointRt(t.xf, t.xfkind, p), XfDirRt(t.xf, t.xfkind, d), sm[k]);
-    } else if constexpr (Q == kQSafeIn) {
-      static_cast<double *>(t.out)[i] = S::SafetyToIn(P, XfPointRt(t.xf, t.xfkind, p));
-    } else {
-      static_cast<double *>(t.out)[i] = S::SafetyToOut(P, p);
-    }
+    Eval64<S, Q>(t, P, i, p, V3{dx[k], dy[k], dz[k]}, sm[k]);
   }
   if (t.count) WaveCount(t.count, n64);
 }
 
-template <class S, int IT>
+// Mixed-precision tile of the compute-heavy classes with lane compaction:
+// every lane runs the FP32 filter on its items and writes the decided ones;
+// the undecided items are listed in LDS and then evaluated in FP64 by the
+// first lanes of the workgroup, so the FP64 code runs on dense waves
+// instead of on the few lanes of each wave that need it.  The inputs of a
+// listed item are read again (from cache) in the second pass.  Called by
+// every thread of the workgroup.
+template <class S, int Q, int IT>
+__device__ __forceinline__ void RunTileCompact(const TaskDev t, const double *blk, uint64_t tile_first) {
+  constexpr int kItems = kBatchBlock * IT;
+  constexpr bool dir = (Q == kQDistIn);
+  __shared__ int count;
+  __shared__ int item[kItems];
+  if (threadIdx.x == 0) count = 0;
+  __syncthreads();
+  const uint64_t i0 = tile_first + (uint64_t)threadIdx.x * IT;
+  const bool full = i0 + IT <= t.n;
+  uint32_t n64 = 0;
+  if (i0 < t.n) {
+    double px[IT], py[IT], pz[IT], dx[IT], dy[IT], dz[IT];
+    LoadAll<IT>(t, i0, full, t.x, px);
+    LoadAll<IT>(t, i0, full, t.y, py);
+    LoadAll<IT>(t, i0, full, t.z, pz);
+    if constexpr (dir) {
+      LoadAll<IT>(t, i0, full, t.dx, dx);
+      LoadAll<IT>(t, i0, full, t.dy, dy);
+      LoadAll<IT>(t, i0, full, t.dz, dz);
+    }
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      const uint64_t i = i0 + k;
+      if (i >= t.n) break;
+      double r;
+      const F3 d32 = dir ? F3{(float)dx[k], (float)dy[k], (float)dz[k]} : F3{0.f, 0.f, 0.f};
+      if (Filter32<S, Q>(t, F3{(float)px[k], (float)py[k], (float)pz[k]}, d32, r)) {
+        WriteDecided<Q>(t, i, r);
+      } else {
+        ++n64;
+        item[atomicAdd(&count, 1)] = (int)(i - tile_first);
+      }
+    }
+  }
+  if (t.count) WaveCount(t.count, n64);
+  __syncthreads();
+  const auto &P = ParamView<S>::Get(blk);
+  const int c = count;
+  for (int j = threadIdx.x; j < c; j += kBatchBlock) {
+    const uint64_t i = tile_first + item[j];
+    const V3 p{LoadOne(t, t.x, i), LoadOne(t, t.y, i), LoadOne(t, t.z, i)};
+    V3 d{0., 0., 0.};
+    if constexpr (dir) d = V3{LoadOne(t, t.dx, i), LoadOne(t, t.dy, i), LoadOne(t, t.dz, i)};
+    Eval64<S, Q>(t, P, i, p, d, (dir && t.sm) ? t.sm[i] : kInfinity);
+  }
+  __syncthreads();  // the list is reused by the next tile of a persistent loop
+}
+
+template <class S, int IT, bool MX>
 __device__ __forceinline__ void RunShape(const TaskDev t, const double *blk, uint64_t i0) {
   switch (t.query) {
-    case kQContains: RunTile<S, kQContains, IT>(t, blk, i0); break;
-    case kQInside: RunTile<S, kQInside, IT>(t, blk, i0); break;
-    case kQDistIn: RunTile<S, kQDistIn, IT>(t, blk, i0); break;
-    case kQDistOut: RunTile<S, kQDistOut, IT>(t, blk, i0); break;
-    case kQPlacedDistOut: RunTile<S, kQPlacedDistOut, IT>(t, blk, i0); break;
-    case kQSafeIn: RunTile<S, kQSafeIn, IT>(t, blk, i0); break;
-    case kQSafeOut: RunTile<S, kQSafeOut, IT>(t, blk, i0); break;
+    case kQContains: RunTile<S, kQContains, IT, MX>(t, blk, i0); break;
+    case kQInside: RunTile<S, kQInside, IT, MX>(t, blk, i0); break;
+    case kQDistIn: RunTile<S, kQDistIn, IT, MX>(t, blk, i0); break;
+    case kQDistOut: RunTile<S, kQDistOut, IT, MX>(t, blk, i0); break;
+    case kQPlacedDistOut: RunTile<S, kQPlacedDistOut, IT, MX>(t, blk, i0); break;
+    case kQSafeIn: RunTile<S, kQSafeIn, IT, MX>(t, blk, i0); break;
+    case kQSafeOut: RunTile<S, kQSafeOut, IT, MX>(t, blk, i0); break;
   }
+}
+
+// The compacted mixed-precision tile of shape S for the filtered queries;
+// false for the others (run per lane).  Uniform over the workgroup.
+template <class S, int IT>
+__device__ __forceinline__ bool RunShapeCompact(const TaskDev t, const double *blk, uint64_t tile_first) {
+  switch (t.query) {
+    case kQContains: RunTileCompact<S, kQContains, IT>(t, blk, tile_first); return true;
+    case kQInside: RunTileCompact<S, kQInside, IT>(t, blk, tile_first); return true;
+    case kQDistIn: RunTileCompact<S, kQDistIn, IT>(t, blk, tile_first); return true;
+  }
+  return false;
 }
 
 // Classes 3 and 4 hold the shapes with variable-length parameter blocks:
 // class 3 those whose block fits in LDS (staged per workgroup), class 4 the
 // others (read from global memory).
-constexpr int kClasses = 5;
+//
+// Mixed-precision tasks form classes of their own (base class +
+// kBaseClasses): their kernels carry the FP32 filter, and the FP64 kernels
+// do not pay its registers.
+constexpr int kBaseClasses = 5;
+constexpr int kClasses = 2 * kBaseClasses;
 constexpr int kStagedClass = 3;
 constexpr int kMaxStageDoubles = 6144;  // 48 KiB of LDS
+constexpr int Base(int c) { return c % kBaseClasses; }
+constexpr bool MixedClass(int c) { return c >= kBaseClasses; }
 
 // Occupancy floor (waves per SIMD) of the staged class's kernel: latency
 // bound, it runs best at 6 waves (a few spilled registers) rather than at
 // the 4 its 121 VGPRs allow; measured 88 -> 100 G queries/s on the batched
 // zoo step (7 waves: 99, 8: 93).  The Torus class measured best unbounded.
-constexpr int ClassWaves(int c) { return c == kStagedClass ? 6 : 1; }
+constexpr int ClassWaves(int c) { return Base(c) == kStagedClass ? 6 : 1; }
 
 constexpr int ClassOf(int type) {
   return (type == VG_CONE || type == VG_SPHERE)           ? 1
@@ -227,7 +328,7 @@ constexpr int ClassOf(int type) {
 
 // Items per lane of class c when the batch asks for `it`: the light class
 // takes it as given, the compute-heavy classes at most 2.
-constexpr int ClassIT(int c, int it) { return c == 0 ? it : (it < 2 ? it : 2); }
+constexpr int ClassIT(int c, int it) { return Base(c) == 0 ? it : (it < 2 ? it : 2); }
 
 // The Polyhedron's distance tasks in the staged class run with lane refill
 // (vg_walk.hpp) over tiles kWalkTileMul times longer, so that each wave
@@ -238,34 +339,64 @@ constexpr bool WalkTask(int type, int query) {
   return type == VG_POLYHEDRON && (query == kQDistIn || query == kQDistOut || query == kQPlacedDistOut);
 }
 constexpr uint64_t TileItems(int c, int it, int type, int query) {
-  return (uint64_t)kBatchBlock * ClassIT(c, it) * ((c == kStagedClass && WalkTask(type, query)) ? kWalkTileMul : 1);
+  return (uint64_t)kBatchBlock * ClassIT(c, it) * ((Base(c) == kStagedClass && WalkTask(type, query)) ? kWalkTileMul : 1);
 }
 
 template <int IT, int CLS>
 __device__ __forceinline__ void RunTask(const TaskDev t, const double *blk, uint64_t i0) {
-  if constexpr (CLS == 0) {
+  constexpr int B = Base(CLS);
+  constexpr bool MX = MixedClass(CLS);
+  if constexpr (B == 0) {
     switch (t.type) {
-      case VG_BOX: RunShape<BoxS, IT>(t, blk, i0); break;
-      case VG_TUBE: RunShape<TubeS, IT>(t, blk, i0); break;
-      case VG_ORB: RunShape<OrbS, IT>(t, blk, i0); break;
-      case VG_TRD: RunShape<TrdS, IT>(t, blk, i0); break;
-      case VG_TRAPEZOID: RunShape<TrapS, IT>(t, blk, i0); break;
-      case VG_PARALLELEPIPED: RunShape<ParaS, IT>(t, blk, i0); break;
-      case VG_PARABOLOID: RunShape<ParabS, IT>(t, blk, i0); break;
+      case VG_BOX: RunShape<BoxS, IT, MX>(t, blk, i0); break;
+      case VG_TUBE: RunShape<TubeS, IT, MX>(t, blk, i0); break;
+      case VG_ORB: RunShape<OrbS, IT, MX>(t, blk, i0); break;
+      case VG_TRD: RunShape<TrdS, IT, MX>(t, blk, i0); break;
+      case VG_TRAPEZOID: RunShape<TrapS, IT, MX>(t, blk, i0); break;
+      case VG_PARALLELEPIPED: RunShape<ParaS, IT, MX>(t, blk, i0); break;
+      case VG_PARABOLOID: RunShape<ParabS, IT, MX>(t, blk, i0); break;
     }
-  } else if constexpr (CLS == 1) {
+  } else if constexpr (B == 1) {
     switch (t.type) {
-      case VG_SPHERE: RunShape<SphereS, IT>(t, blk, i0); break;
-      case VG_CONE: RunShape<ConeS, IT>(t, blk, i0); break;
+      case VG_SPHERE: RunShape<SphereS, IT, MX>(t, blk, i0); break;
+      case VG_CONE: RunShape<ConeS, IT, MX>(t, blk, i0); break;
     }
-  } else if constexpr (CLS == 2) {
-    if (t.type == VG_TORUS) RunShape<TorusS, IT>(t, blk, i0);
+  } else if constexpr (B == 2) {
+    if (t.type == VG_TORUS) RunShape<TorusS, IT, MX>(t, blk, i0);
   } else {  // classes 3 and 4
     switch (t.type) {
-      case VG_POLYCONE: RunShape<PolyconeS, IT>(t, blk, i0); break;
-      case VG_POLYHEDRON: RunShape<PolyhedronS, IT>(t, blk, i0); break;
+      case VG_POLYCONE: RunShape<PolyconeS, IT, MX>(t, blk, i0); break;
+      case VG_POLYHEDRON: RunShape<PolyhedronS, IT, MX>(t, blk, i0); break;
     }
   }
+}
+
+// The compacted mixed-precision tile of a compute-heavy class (not the
+// light one); false when the task's query runs per lane.
+template <int IT, int CLS>
+__device__ __forceinline__ bool RunTaskCompact(const TaskDev t, const double *blk, uint64_t tile_first) {
+  constexpr int B = Base(CLS);
+  if constexpr (B == 0) {
+    switch (t.type) {
+      case VG_BOX: return RunShapeCompact<BoxS, IT>(t, blk, tile_first);
+      case VG_TUBE: return RunShapeCompact<TubeS, IT>(t, blk, tile_first);
+      case VG_ORB: return RunShapeCompact<OrbS, IT>(t, blk, tile_first);
+      case VG_TRD: return RunShapeCompact<TrdS, IT>(t, blk, tile_first);
+      case VG_TRAPEZOID: return RunShapeCompact<TrapS, IT>(t, blk, tile_first);
+      case VG_PARALLELEPIPED: return RunShapeCompact<ParaS, IT>(t, blk, tile_first);
+      case VG_PARABOLOID: return RunShapeCompact<ParabS, IT>(t, blk, tile_first);
+    }
+    return false;
+  } else if constexpr (B == 1) {
+    if (t.type == VG_SPHERE) return RunShapeCompact<SphereS, IT>(t, blk, tile_first);
+    return RunShapeCompact<ConeS, IT>(t, blk, tile_first);
+  } else if constexpr (B == 2) {
+    return RunShapeCompact<TorusS, IT>(t, blk, tile_first);
+  } else if constexpr (B >= 3) {
+    if (t.type == VG_POLYCONE) return RunShapeCompact<PolyconeS, IT>(t, blk, tile_first);
+    return RunShapeCompact<PolyhedronS, IT>(t, blk, tile_first);
+  }
+  return false;
 }
 
 // DistanceToIn of one tile with lane compaction (the Torus, whose exact
@@ -273,7 +404,7 @@ __device__ __forceinline__ void RunTask(const TaskDev t, const double *blk, uint
 // writes kInfinity for the rejected ones and appends the others to a
 // workgroup list in LDS; then the list is processed by the first lanes, so
 // the quartic runs on dense waves.  Called by every thread of the workgroup.
-template <class S, int IT>
+template <class S, int IT, bool MX>
 __device__ __forceinline__ void RunTileCompactDistIn(const TaskDev t, const double *blk, uint64_t tile_first) {
   constexpr int kItems = kBatchBlock * IT;
   __shared__ int count;
@@ -298,8 +429,8 @@ __device__ __forceinline__ void RunTileCompactDistIn(const TaskDev t, const doub
       const uint64_t i = i0 + k;
       if (i >= t.n) break;
       double r;
-      if (t.in_type == VG_MIXED && Filter32<kQDistIn>(t, F3{(float)px[k], (float)py[k], (float)pz[k]},
-                                                      F3{(float)dx[k], (float)dy[k], (float)dz[k]}, r)) {
+      if (MX && Filter32<S, kQDistIn>(t, F3{(float)px[k], (float)py[k], (float)pz[k]},
+                                   F3{(float)dx[k], (float)dy[k], (float)dz[k]}, r)) {
         static_cast<double *>(t.out)[i] = r;
         continue;
       }
@@ -331,15 +462,12 @@ __device__ __forceinline__ void RunTileCompactDistIn(const TaskDev t, const doub
   }
 }
 
-__device__ __forceinline__ double LoadOne(const TaskDev &t, const void *a, uint64_t i) {
-  return t.in_type != VG_F64 ? (double)static_cast<const float *>(a)[i] : static_cast<const double *>(a)[i];
-}
 
 // Items [first, first + count) of a walk task with lane refill: each of the
 // workgroup's waves takes a quarter of the range.  Local says the inputs are
 // in the shape's frame already (DistanceToOut).  Called by every thread of
 // the workgroup.
-template <class S, class Ops, bool Local>
+template <class S, class Ops, bool Local, bool MX>
 __device__ __forceinline__ void RunTileWalk(const TaskDev &t, const double *blk, uint64_t first, uint64_t count) {
   constexpr int kWaves = kBatchBlock / 64;
   const uint64_t per = (count + kWaves - 1) / kWaves;
@@ -348,14 +476,14 @@ __device__ __forceinline__ void RunTileWalk(const TaskDev &t, const double *blk,
   const uint64_t begin = b < lim ? b : lim;
   const uint64_t end = begin + per < lim ? begin + per : lim;
   double *out = static_cast<double *>(t.out);
-  const bool filt = !Local && t.in_type == VG_MIXED && t.query == kQDistIn;
+  const bool filt = MX && !Local && t.query == kQDistIn;
   uint32_t n64 = 0;
   WalkRefill<Ops>(
       ParamView<S>::Get(blk), begin, end,
       [&](uint64_t i, V3 &p, V3 &d, double &sm, double &r) {
         p = V3{LoadOne(t, t.x, i), LoadOne(t, t.y, i), LoadOne(t, t.z, i)};
         d = V3{LoadOne(t, t.dx, i), LoadOne(t, t.dy, i), LoadOne(t, t.dz, i)};
-        if (filt && Filter32<kQDistIn>(t, F3{(float)p.x, (float)p.y, (float)p.z}, F3{(float)d.x, (float)d.y, (float)d.z}, r))
+        if (filt && Filter32<S, kQDistIn>(t, F3{(float)p.x, (float)p.y, (float)p.z}, F3{(float)d.x, (float)d.y, (float)d.z}, r))
           return true;
         ++n64;
         if (!Local) {
@@ -395,10 +523,15 @@ __device__ __forceinline__ int FindTask(const TaskDev *tasks, int ntasks, uint64
 template <int IT, int CLS>
 __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(ClassWaves(CLS)))) void BatchKernel(const TaskDev *__restrict__ tasks, int ntasks,
                                                            uint64_t ntiles, const double *__restrict__ params) {
-  if constexpr (CLS == 0) {
+  constexpr int B = Base(CLS);
+  constexpr bool MX = MixedClass(CLS);
+  if constexpr (B == 0) {
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
       const int k = FindTask(tasks, ntasks, tile);
       const TaskDev t = tasks[k];
+      if constexpr (MX) {
+        if (RunTaskCompact<IT, CLS>(t, params + t.param_off, (tile - t.tile_begin) * kBatchBlock * IT)) continue;
+      }
       const uint64_t i0 = ((tile - t.tile_begin) * kBatchBlock + threadIdx.x) * IT;
       if (i0 < t.n) RunTask<IT, CLS>(t, params + t.param_off, i0);
     }
@@ -407,32 +540,36 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(Cla
     if (tile >= ntiles) return;
     const int k = FindTask(tasks, ntasks, tile);
     const TaskDev t = tasks[k];
-    const uint64_t i0 = ((tile - t.tile_begin) * kBatchBlock + threadIdx.x) * IT;
-    if constexpr (CLS == kStagedClass) {
+    const uint64_t tile_first = (tile - t.tile_begin) * kBatchBlock * IT;
+    const uint64_t i0 = tile_first + threadIdx.x * IT;
+    const double *blk = params + t.param_off;
+    if constexpr (B == kStagedClass) {
       extern __shared__ __align__(16) double stage[];
       Stage(stage, params + t.param_off, t.param_len);
       __syncthreads();
+      blk = stage;
       if (WalkTask(t.type, t.query)) {  // uniform over the workgroup
         const uint64_t per = TileItems(CLS, IT, t.type, t.query), first = (tile - t.tile_begin) * per;
         if (t.query == kQDistIn) {
-          RunTileWalk<PolyhedronS, PolyhedronS::DistInOps, false>(t, stage, first, per);
+          RunTileWalk<PolyhedronS, PolyhedronS::DistInOps, false, MX>(t, stage, first, per);
         } else if (t.query == kQDistOut) {
-          RunTileWalk<PolyhedronS, PolyhedronS::DistOutOps, true>(t, stage, first, per);
+          RunTileWalk<PolyhedronS, PolyhedronS::DistOutOps, true, MX>(t, stage, first, per);
         } else {
-          RunTileWalk<PolyhedronS, PolyhedronS::DistOutOps, false>(t, stage, first, per);
+          RunTileWalk<PolyhedronS, PolyhedronS::DistOutOps, false, MX>(t, stage, first, per);
         }
-      } else if (i0 < t.n) {
-        RunTask<IT, CLS>(t, stage, i0);
+        return;
       }
-    } else if constexpr (CLS == 2) {
-      if (t.query == kQDistIn) {  // uniform over the workgroup
-        RunTileCompactDistIn<TorusS, IT>(t, params + t.param_off, (tile - t.tile_begin) * kBatchBlock * IT);
-      } else if (i0 < t.n) {
-        RunTask<IT, CLS>(t, params + t.param_off, i0);
-      }
-    } else {
-      if (i0 < t.n) RunTask<IT, CLS>(t, params + t.param_off, i0);
     }
+    if constexpr (B == 2) {
+      if (t.query == kQDistIn) {  // uniform over the workgroup
+        RunTileCompactDistIn<TorusS, IT, MX>(t, blk, tile_first);
+        return;
+      }
+    }
+    if constexpr (MX) {
+      if (RunTaskCompact<IT, CLS>(t, blk, tile_first)) return;
+    }
+    if (i0 < t.n) RunTask<IT, CLS>(t, blk, i0);
   }
 }
 
@@ -441,7 +578,7 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(Cla
 template <int IT, int CLS>
 __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(ClassWaves(CLS)))) void TaskKernel(const TaskDev t, const double *__restrict__ params) {
   const uint64_t i0 = ((uint64_t)blockIdx.x * kBatchBlock + threadIdx.x) * IT;
-  if constexpr (CLS == kStagedClass) {
+  if constexpr (Base(CLS) == kStagedClass) {
     extern __shared__ __align__(16) double stage[];
     if (t.param_len <= kMaxStageDoubles) {
       Stage(stage, params + t.param_off, t.param_len);
@@ -463,8 +600,8 @@ struct vg_batch {
   const vg_geometry *g = nullptr;
   std::vector<vg_query_task> tasks;
   std::vector<TaskDev> host;                  // sorted by weight class
-  int first[vga::kClasses + 1] = {0, 0, 0, 0, 0, 0};  // class c owns host[first[c] .. first[c+1])
-  uint64_t ntiles[vga::kClasses] = {0, 0, 0, 0, 0};   // tiles per class for the current IT
+  int first[vga::kClasses + 1] = {};   // class c owns host[first[c] .. first[c+1])
+  uint64_t ntiles[vga::kClasses] = {};  // tiles per class for the current IT
   int stage_len = 0;                               // LDS doubles for the staged class
   // One device task table per items-per-lane value (1, 2, 4), each written
   // once at create: changing the launch shape only selects another table,
@@ -478,9 +615,9 @@ struct vg_batch {
   // Classes run concurrently: the first non-empty class on the caller's
   // stream, the others on side streams forked from and joined back to it
   // with events (a pattern hipGraph capture records as parallel branches).
-  hipStream_t side[vga::kClasses - 1] = {nullptr, nullptr, nullptr, nullptr};
+  hipStream_t side[vga::kClasses - 1] = {};
   hipEvent_t fork = nullptr;
-  hipEvent_t join[vga::kClasses - 1] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t join[vga::kClasses - 1] = {};
   // Per-class timing (vg_batch_set_timing): a ring of event pairs recorded
   // on the stream each class kernel is launched on, around that kernel.
   static constexpr int kRing = 64;
@@ -489,8 +626,8 @@ struct vg_batch {
   hipEvent_t t0[vga::kClasses][kRing] = {};
   hipEvent_t t1[vga::kClasses][kRing] = {};
   bool recorded[vga::kClasses][kRing] = {};
-  uint64_t class_items[vga::kClasses] = {0, 0, 0, 0, 0};
-  uint64_t class_bytes[vga::kClasses] = {0, 0, 0, 0, 0};
+  uint64_t class_items[vga::kClasses] = {};
+  uint64_t class_bytes[vga::kClasses] = {};
 };
 
 namespace {
@@ -518,9 +655,29 @@ int QueryWeight(int q) {
   return q == kQDistIn ? 3 : (q == kQContains || q == kQInside) ? 1 : 2;
 }
 
+// Mixed-precision policy (vg_set_mixed_policy).  On gfx950 an FP64 FMA
+// issues at the rate of an unpacked FP32 one, so the FP32 filter pays only
+// where a decided lane skips long FP64 work; measured per (shape, query) on
+// the mixed zoo workload (filter time / FP32-storage time, single-task
+// batches of 2^21 queries): Polycone DistanceToIn 0.94; every other pair
+// 0.99-3.9 (classifications 1.4-3.9: the FP64 Contains / Inside of these
+// shapes is a handful of comparisons, cheaper than the FP32 bound and its
+// margin).  Under the measured policy a mixed task without a paying filter
+// runs in the FP64 class kernels as FP32 storage (identical results).
+int g_mixed_policy = VG_MIXED_POLICY_MEASURED;
+
+bool FilterPays(int type, int query) { return type == VG_POLYCONE && query == kQDistIn; }
+
+bool Filtered(const TaskDev &t) {
+  if (t.in_type != VG_MIXED) return false;
+  if (t.query != kQContains && t.query != kQInside && t.query != kQDistIn) return false;
+  return g_mixed_policy == VG_MIXED_POLICY_ALL || FilterPays(t.type, t.query);
+}
+
 int TaskClass(const TaskDev &t) {
-  const int c = ClassOf(t.type);
-  return (c == kStagedClass && t.param_len > kMaxStageDoubles) ? kStagedClass + 1 : c;
+  int c = ClassOf(t.type);
+  if (c == kStagedClass && t.param_len > kMaxStageDoubles) c = kStagedClass + 1;
+  return c + (Filtered(t) ? kBaseClasses : 0);
 }
 
 // Fills tile_begin (numbered within each class) for items per lane `it`.
@@ -539,7 +696,7 @@ void Schedule(vg_batch *b) {
 
 template <int IT, int CLS>
 void LaunchClass(const vg_batch *b, unsigned grid, hipStream_t s) {
-  const size_t lds = CLS == kStagedClass ? (size_t)b->stage_len * sizeof(double) : 0;
+  const size_t lds = Base(CLS) == kStagedClass ? (size_t)b->stage_len * sizeof(double) : 0;
   hipLaunchKernelGGL((BatchKernel<IT, CLS>), dim3(grid), dim3(kBatchBlock), lds, s, b->d_tasks + b->first[CLS],
                      b->first[CLS + 1] - b->first[CLS], b->ntiles[CLS], b->g->d_params);
 }
@@ -590,21 +747,23 @@ int Launch(vg_batch *b, hipStream_t s) {
       if (rc) break;
     }
     first = false;
-    const unsigned grid = (c == 0 && b->blocks) ? (unsigned)std::min<uint64_t>(b->blocks, nt) : (unsigned)nt;
+    const unsigned grid = (Base(c) == 0 && b->blocks) ? (unsigned)std::min<uint64_t>(b->blocks, nt) : (unsigned)nt;
     if (b->timing) {
       rc = CheckHip(hipEventRecord(b->t0[c][slot], cs), "batch timing");
       if (rc) break;
     }
-    if (c == 0)
-      LaunchIT<0>(b, grid, cs);
-    else if (c == 1)
-      LaunchIT<1>(b, grid, cs);
-    else if (c == 2)
-      LaunchIT<2>(b, grid, cs);
-    else if (c == 3)
-      LaunchIT<3>(b, grid, cs);
-    else
-      LaunchIT<4>(b, grid, cs);
+    switch (c) {
+      case 0: LaunchIT<0>(b, grid, cs); break;
+      case 1: LaunchIT<1>(b, grid, cs); break;
+      case 2: LaunchIT<2>(b, grid, cs); break;
+      case 3: LaunchIT<3>(b, grid, cs); break;
+      case 4: LaunchIT<4>(b, grid, cs); break;
+      case 5: LaunchIT<5>(b, grid, cs); break;
+      case 6: LaunchIT<6>(b, grid, cs); break;
+      case 7: LaunchIT<7>(b, grid, cs); break;
+      case 8: LaunchIT<8>(b, grid, cs); break;
+      default: LaunchIT<9>(b, grid, cs); break;
+    }
     rc = CheckHip(hipGetLastError(), "batch launch");
     if (!rc && b->timing) {
       rc = CheckHip(hipEventRecord(b->t1[c][slot], cs), "batch timing");
@@ -726,6 +885,28 @@ int MakeTask(const vg_geometry *g, const vg_query_task &q, int i, TaskDev &t) {
   return VG_OK;
 }
 
+template <int CLS>
+void LaunchTaskIT(int it, dim3 gr, size_t lds, hipStream_t s, const TaskDev &t, const double *params) {
+  const size_t l = Base(CLS) == kStagedClass ? lds : 0;
+  if (it == 2)
+    hipLaunchKernelGGL((TaskKernel<2, CLS>), gr, dim3(kBatchBlock), l, s, t, params);
+  else
+    hipLaunchKernelGGL((TaskKernel<1, CLS>), gr, dim3(kBatchBlock), l, s, t, params);
+}
+
+void LaunchTask(int cls, int it, dim3 gr, size_t lds, hipStream_t s, const TaskDev &t, const double *params) {
+  switch (cls) {
+    case 0: LaunchTaskIT<0>(it, gr, lds, s, t, params); break;
+    case 1: LaunchTaskIT<1>(it, gr, lds, s, t, params); break;
+    case 2: LaunchTaskIT<2>(it, gr, lds, s, t, params); break;
+    case 3: LaunchTaskIT<3>(it, gr, lds, s, t, params); break;
+    case 5: LaunchTaskIT<5>(it, gr, lds, s, t, params); break;
+    case 6: LaunchTaskIT<6>(it, gr, lds, s, t, params); break;
+    case 7: LaunchTaskIT<7>(it, gr, lds, s, t, params); break;
+    default: LaunchTaskIT<8>(it, gr, lds, s, t, params); break;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -761,8 +942,8 @@ int vg_batch_create(const vg_geometry *g, const vg_query_task *tasks, int32_t nt
     while (k < (int)b->host.size() && TaskClass(b->host[k]) < c) ++k;
     b->first[c] = k;
   }
-  for (int k = b->first[kStagedClass]; k < b->first[kStagedClass + 1]; ++k)
-    b->stage_len = std::max(b->stage_len, b->host[k].param_len);
+  for (int c : {kStagedClass, kStagedClass + kBaseClasses})
+    for (int k = b->first[c]; k < b->first[c + 1]; ++k) b->stage_len = std::max(b->stage_len, b->host[k].param_len);
   for (int c = 0; c < kClasses; ++c)
     for (int k = b->first[c]; k < b->first[c + 1]; ++k) {
       b->class_items[c] += b->host[k].n;
@@ -790,19 +971,19 @@ int vg_query(const vg_geometry *g, const vg_query_task *task, void *stream) {
   const int it = t.vec_ok ? 2 : 1;
   const uint64_t blocks = (t.n + (uint64_t)it * kBatchBlock - 1) / ((uint64_t)it * kBatchBlock);
   const hipStream_t s = (hipStream_t)stream;
-  const dim3 gr((unsigned)blocks), bl(kBatchBlock);
   const size_t lds = t.param_len <= kMaxStageDoubles ? (size_t)t.param_len * sizeof(double) : 0;
-  switch (ClassOf(t.type) * 2 + (it == 2 ? 1 : 0)) {
-    case 0: hipLaunchKernelGGL((TaskKernel<1, 0>), gr, bl, 0, s, t, g->d_params); break;
-    case 1: hipLaunchKernelGGL((TaskKernel<2, 0>), gr, bl, 0, s, t, g->d_params); break;
-    case 2: hipLaunchKernelGGL((TaskKernel<1, 1>), gr, bl, 0, s, t, g->d_params); break;
-    case 3: hipLaunchKernelGGL((TaskKernel<2, 1>), gr, bl, 0, s, t, g->d_params); break;
-    case 4: hipLaunchKernelGGL((TaskKernel<1, 2>), gr, bl, 0, s, t, g->d_params); break;
-    case 5: hipLaunchKernelGGL((TaskKernel<2, 2>), gr, bl, 0, s, t, g->d_params); break;
-    case 6: hipLaunchKernelGGL((TaskKernel<1, 3>), gr, bl, lds, s, t, g->d_params); break;
-    default: hipLaunchKernelGGL((TaskKernel<2, 3>), gr, bl, lds, s, t, g->d_params); break;
-  }
+  // Class 4 (blocks too large for LDS) runs the class-3 code, which reads
+  // the block from global memory when it does not fit.
+  const int cls = std::min(ClassOf(t.type), kStagedClass) + (Filtered(t) ? kBaseClasses : 0);
+  LaunchTask(cls, it, dim3((unsigned)blocks), lds, s, t, g->d_params);
   return CheckHip(hipGetLastError(), "query launch");
+}
+
+int vg_set_mixed_policy(int32_t policy) {
+  if (policy != VG_MIXED_POLICY_MEASURED && policy != VG_MIXED_POLICY_ALL)
+    return SetError(VG_ERR_INVALID, "unknown mixed-precision policy");
+  g_mixed_policy = policy;
+  return VG_OK;
 }
 
 int vg_batch_run(vg_batch *b, void *stream) {

@@ -15,7 +15,8 @@
 //   Paraboloid     a, b, dz  (inside: z >= a rho^2 + b, |z| <= dz)
 // and in the side array (B32::off):
 //   Polycone       nsec, wedge[5], then per section zlo, zhi, cone[7]
-//   Polyhedron     nseg, then per segment zlo, zhi, nout, nin, nout + nin
+//   Polyhedron     nseg, the offset of each segment record (relative to the
+//                  start), then per segment zlo, zhi, nout, nin, nout + nin
 //                  planes (unit normal, offset), oriented with the z axis at
 //                  the segment's mid-plane on the negative side (outer planes:
 //                  the solid side; inner planes: the hole's side)
@@ -215,8 +216,11 @@ void BuildBound32(int type, const double *blk, const Xf &xf, const double lo[3],
       const int n = PolyhedronS::NSeg(q);
       b.off = (int32_t)side.size();
       side.push_back((float)n);
+      const size_t offs = side.size();
+      side.resize(offs + n, 0.f);  // record offsets, relative to b.off
       double dmax = 0;
       for (int k = 0; k < n; ++k) {
+        side[offs + k] = (float)(side.size() - b.off);
         const auto sg = PolyhedronS::GetSeg(q, k);
         const double zlo = PolyhedronS::Z(q, k), zhi = PolyhedronS::Z(q, k + 1);
         const V3 ref{0., 0., 0.5 * (zlo + zhi)};

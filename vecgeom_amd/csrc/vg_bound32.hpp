@@ -135,12 +135,13 @@ VGA_HD void Cone32(const float *c, float rho, float z, float &so, float &si) {
   }
 }
 
-// Bounds of a shape at local point l.  side: variable-length data.
-VGA_HD void ShapeBound32(const B32 &b, const float *side, F3 l, float &so, float &si) {
+// Bounds of a shape of kind K at local point l.  side: variable-length data.
+template <int K>
+VGA_HD void BoundOf(const B32 &b, const float *side, F3 l, float &so, float &si) {
   const float *s = b.s;
   so = -__builtin_huge_valf();
   si = __builtin_huge_valf();
-  switch (b.kind) {
+  switch (K) {
     case VG_BOX: {
       const float ox = FAbs(l.x) - s[0], oy = FAbs(l.y) - s[1], oz = FAbs(l.z) - s[2];
       so = FMax(ox, FMax(oy, oz));
@@ -250,30 +251,52 @@ VGA_HD void ShapeBound32(const B32 &b, const float *side, F3 l, float &so, float
       break;
     }
     case VG_POLYCONE: {  // side: nsec, wedge[5], then per section zlo, zhi, cone[7]
+      // Union over the sections.  Sections are ordered along z, and a
+      // section's bounds are at least its slab distance max(zlo - z, z - zhi)
+      // away (so) / at most minus it (si): starting from the section nearest
+      // in z, sections are visited outwards while their slab distance can
+      // still lower so; only sections whose slab holds z can raise si.
       const float *q = side + b.off;
       const int n = (int)q[0];
       const float rho = FSqrt(l.x * l.x + l.y * l.y);
-      for (int k = 0; k < n; ++k) {
+      auto sec = [&](int k, float &o, float &in) {
         const float *c = q + 6 + 9 * k;
-        float o = FMax(c[0] - l.z, l.z - c[1]), in = -o;
+        o = FMax(c[0] - l.z, l.z - c[1]);
+        in = -o;
         Cone32(c + 2, rho, l.z, o, in);
-        so = k == 0 ? o : FMin(so, o);
-        si = k == 0 ? in : FMax(si, in);
+      };
+      int k0 = 0;
+      while (k0 + 1 < n && l.z > q[6 + 9 * k0 + 1]) ++k0;
+      sec(k0, so, si);
+      for (int k = k0 - 1; k >= 0; --k) {  // below: slab distance z - zhi
+        if (l.z - q[6 + 9 * k + 1] >= so) break;
+        float o, in;
+        sec(k, o, in);
+        so = FMin(so, o);
+        si = FMax(si, in);
+      }
+      for (int k = k0 + 1; k < n; ++k) {  // above: slab distance zlo - z
+        if (q[6 + 9 * k] - l.z >= so) break;
+        float o, in;
+        sec(k, o, in);
+        so = FMin(so, o);
+        si = FMax(si, in);
       }
       Wedge32(q + 1, l, so, si);
       break;
     }
-    case VG_POLYHEDRON: {  // side: nseg, then per segment zlo, zhi, nout, nin, planes (a, b, c, d)
+    case VG_POLYHEDRON: {  // side: nseg, then offsets of the segment records, then the records
+      // Segment record: zlo, zhi, nout, nin, nout + nin planes.  Visited as
+      // the Polycone's sections are.
       const float *q = side + b.off;
       const int n = (int)q[0];
-      int at = 1;
-      for (int k = 0; k < n; ++k) {
-        const float *g = q + at;
+      auto seg = [&](int k, float &o, float &in) {
+        const float *g = q + (int)q[1 + k];
         const int no = (int)g[2], ni = (int)g[3];
-        float o = FMax(g[0] - l.z, l.z - g[1]);
+        o = FMax(g[0] - l.z, l.z - g[1]);
         const float *pl = g + 4;
         for (int i = 0; i < no; ++i, pl += 4) o = FMax(o, pl[0] * l.x + pl[1] * l.y + pl[2] * l.z + pl[3]);
-        float in = -o;
+        in = -o;
         if (ni > 0) {
           // Inner planes are oriented with the hole on their negative side:
           // outside the hole when some plane is positive (depth of the
@@ -283,9 +306,25 @@ VGA_HD void ShapeBound32(const B32 &b, const float *side, F3 l, float &so, float
           o = FMax(o, -hmax);
           in = FMin(in, hmax);
         }
-        at += 4 + 4 * (no + ni);
-        so = k == 0 ? o : FMin(so, o);
-        si = k == 0 ? in : FMax(si, in);
+      };
+      auto zlo = [&](int k) { return q[(int)q[1 + k]]; };
+      auto zhi = [&](int k) { return q[(int)q[1 + k] + 1]; };
+      int k0 = 0;
+      while (k0 + 1 < n && l.z > zhi(k0)) ++k0;
+      seg(k0, so, si);
+      for (int k = k0 - 1; k >= 0; --k) {
+        if (l.z - zhi(k) >= so) break;
+        float o, in;
+        seg(k, o, in);
+        so = FMin(so, o);
+        si = FMax(si, in);
+      }
+      for (int k = k0 + 1; k < n; ++k) {
+        if (zlo(k) - l.z >= so) break;
+        float o, in;
+        seg(k, o, in);
+        so = FMin(so, o);
+        si = FMax(si, in);
       }
       break;
     }
@@ -294,21 +333,55 @@ VGA_HD void ShapeBound32(const B32 &b, const float *side, F3 l, float &so, float
   }
 }
 
+// Shapes present in a mask of kinds (bit k: kind k may occur): a kernel that
+// knows its shapes compiles only their bounds.
+constexpr uint32_t kAllBoundKinds = 0xFFFFFFFFu;
+
+#define VGA_BOUND_CASE(K)                                 \
+  case K:                                                 \
+    if constexpr ((MASK >> K) & 1u) BoundOf<K>(b, side, l, so, si); \
+    break;
+
+template <uint32_t MASK = kAllBoundKinds>
+VGA_HD void ShapeBound32(const B32 &b, const float *side, F3 l, float &so, float &si) {
+  so = -__builtin_huge_valf();
+  si = __builtin_huge_valf();
+  switch (b.kind) {
+    VGA_BOUND_CASE(VG_BOX)
+    VGA_BOUND_CASE(VG_ORB)
+    VGA_BOUND_CASE(VG_TUBE)
+    VGA_BOUND_CASE(VG_CONE)
+    VGA_BOUND_CASE(VG_SPHERE)
+    VGA_BOUND_CASE(VG_TORUS)
+    VGA_BOUND_CASE(VG_TRD)
+    VGA_BOUND_CASE(VG_TRAPEZOID)
+    VGA_BOUND_CASE(VG_PARALLELEPIPED)
+    VGA_BOUND_CASE(VG_PARABOLOID)
+    VGA_BOUND_CASE(VG_POLYCONE)
+    VGA_BOUND_CASE(VG_POLYHEDRON)
+    default:
+      break;
+  }
+}
+#undef VGA_BOUND_CASE
+
 // so, si and the margin of a mother-frame point.  For kBoundNone every
 // point is undecided (so = si = -inf).
+template <uint32_t MASK = kAllBoundKinds>
 VGA_HD void Bound32(const B32 &b, const float *side, F3 p, float &so, float &si, float &m) {
   m = Margin32(b, p);
   if (b.kind == kBoundNone) {
     so = si = -__builtin_huge_valf();
     return;
   }
-  ShapeBound32(b, side, Local32(b, p), so, si);
+  ShapeBound32<MASK>(b, side, Local32(b, p), so, si);
 }
 
 // Decision of the FP32 filter: 0 inside, 2 outside, -1 undecided.
+template <uint32_t MASK = kAllBoundKinds>
 VGA_HD int Classify32(const B32 &b, const float *side, F3 p) {
   float so, si, m;
-  Bound32(b, side, p, so, si, m);
+  Bound32<MASK>(b, side, p, so, si, m);
   if (so > m) return kOutside;
   if (si > m) return kInside;
   return -1;

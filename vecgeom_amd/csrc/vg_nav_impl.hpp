@@ -319,7 +319,7 @@ template <uint32_t M = kAllShapes>
 VGA_NAV bool PlacedContains(const NavDev &g, int pv, V3 p, V3 &local, bool mixed = false) {
   int c = -1;
   if (mixed) {
-    c = Classify32(g.Bound(pv), g.b32side, F3{(float)p.x, (float)p.y, (float)p.z});
+    c = Classify32<M>(g.Bound(pv), g.b32side, F3{(float)p.x, (float)p.y, (float)p.z});
     if (c == kOutside) return false;
   }
   const PlacedRec &r = g.Pv(pv);
@@ -493,7 +493,7 @@ VGA_NAV int AbboxCandidates(const NavDev &g, int db, int de, V3 lp, V3 ld, doubl
     const int cand = g.Daughter(k);
     if (Mixed) {
       float so, si, m;
-      Bound32(g.Bound(cand), g.b32side, lp32, so, si, m);
+      Bound32<M>(g.Bound(cand), g.b32side, lp32, so, si, m);
       if ((double)so - (double)m > Max(best, 0.) + 1e-6) return true;
     }
     const PlacedRec &r = g.Pv(cand);
