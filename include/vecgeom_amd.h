@@ -317,6 +317,18 @@ int vg_batch_set_timing(vg_batch *b, int32_t enable);
 int vg_batch_class_stats(vg_batch *b, int32_t cls, uint64_t *items, uint64_t *bytes, int32_t *runs, float *mean_ms);
 int vg_batch_free(vg_batch *b);
 
+/* The tasks of a batch run once on `stream` without allocating anything:
+ * no hipMalloc / hipFree, no streams or events created; the task table goes
+ * to a static device array (one per device, 256 tasks at most -- more is
+ * VG_ERR_CAPACITY before anything is launched) and the class kernels run one
+ * after another on the stream.  Results equal vg_batch_create + vg_batch_run
+ * on the same tasks.  Calls on one device share its table: calls whose work
+ * overlaps on one device (on different streams, or from several threads)
+ * race, and each call rewrites the whole table, so nothing carries over.
+ * No counterpart in the reference: for callers that cannot allocate (code
+ * under graph capture, serving loops, harnesses that forbid hipMalloc). */
+int vg_batch_run_noalloc(const vg_geometry *g, const vg_query_task *tasks, int32_t ntasks, void *stream);
+
 /* -------------------------------------------------------------- navigation */
 /*
  * Navigation states of a basket of tracks, structure-of-arrays in device

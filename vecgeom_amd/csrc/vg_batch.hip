@@ -38,6 +38,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -907,9 +908,107 @@ void LaunchTask(int cls, int it, dim3 gr, size_t lds, hipStream_t s, const TaskD
   }
 }
 
+// ------------------------------------------------ the no-allocation batch
+// vg_batch_run_noalloc keeps its task table in this static device array
+// instead of memory allocated per batch.  The array's name is a host-side
+// stand-in; host code uses only the address hipGetSymbolAddress returns for
+// it on each device.  Every call on a device shares that device's array.
+constexpr int kScratchTasks = 256;
+static __device__ __align__(256) TaskDev scratch[kScratchTasks];
+
+constexpr int kMaxDevices = 64;
+TaskDev *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+
+// The device address of `scratch` on the current device, looked up once per
+// device.
+int ScratchPtr(TaskDev **out) {
+  int dev = -1;
+  int rc = CheckHip(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  if (dev < 0 || dev >= kMaxDevices) return SetError(VG_ERR_STATE, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    rc = CheckHip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (rc) return rc;
+    g_scratch_ptr[dev] = static_cast<TaskDev *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return VG_OK;
+}
+
+template <int CLS>
+void LaunchNoAlloc(const TaskDev *tasks, int ntasks, uint64_t ntiles, int stage_len, const double *params,
+                   hipStream_t s) {
+  constexpr int IT = ClassIT(CLS, 2);
+  const size_t lds = Base(CLS) == kStagedClass ? (size_t)stage_len * sizeof(double) : 0;
+  hipLaunchKernelGGL((BatchKernel<IT, CLS>), dim3((unsigned)ntiles), dim3(kBatchBlock), lds, s, tasks, ntasks, ntiles,
+                     params);
+}
+
 }  // namespace
 
 extern "C" {
+
+int vg_batch_run_noalloc(const vg_geometry *g, const vg_query_task *tasks, int32_t ntasks, void *stream) {
+  if (!g || ntasks < 0 || (ntasks > 0 && !tasks)) return SetError(VG_ERR_INVALID, "null argument");
+  if (ntasks > kScratchTasks)
+    return SetError(VG_ERR_CAPACITY, "vg_batch_run_noalloc takes at most " + std::to_string(kScratchTasks) +
+                                         " tasks (" + std::to_string(ntasks) + " given)");
+  int rc = EnsureDevice();
+  if (rc) return rc;
+  TaskDev *dev = nullptr;
+  if ((rc = ScratchPtr(&dev))) return rc;
+  // Host staging of the table: a fixed per-thread array (no allocation).
+  static thread_local TaskDev host[kScratchTasks];
+  int n = 0;
+  for (int i = 0; i < ntasks; ++i) {
+    TaskDev t;
+    if ((rc = MakeTask(g, tasks[i], i, t))) return rc;
+    if (t.n > 0) host[n++] = t;
+  }
+  if (n == 0) return VG_OK;
+  std::stable_sort(host, host + n, [](const TaskDev &a, const TaskDev &c) { return TaskClass(a) < TaskClass(c); });
+  int first[kClasses + 1];
+  uint64_t ntiles[kClasses];
+  int stage_len = 0;
+  for (int c = 0, k = 0; c < kClasses; ++c) {
+    first[c] = k;
+    uint64_t tile = 0;
+    for (; k < n && TaskClass(host[k]) == c; ++k) {
+      const uint64_t per = TileItems(c, 2, host[k].type, host[k].query);
+      host[k].tile_begin = tile;
+      tile += (host[k].n + per - 1) / per;
+      if (Base(c) == kStagedClass) stage_len = std::max(stage_len, host[k].param_len);
+    }
+    ntiles[c] = tile;
+  }
+  first[kClasses] = n;
+  const hipStream_t s = (hipStream_t)stream;
+  // A pageable-memory copy returns once the host array has been staged, so
+  // the array may be refilled by the next call.
+  rc = CheckHip(hipMemcpyAsync(dev, host, n * sizeof(TaskDev), hipMemcpyHostToDevice, s), "scratch task table upload");
+  for (int c = 0; c < kClasses && !rc; ++c) {
+    if (!ntiles[c]) continue;
+    const TaskDev *tt = dev + first[c];
+    const int cnt = first[c + 1] - first[c];
+    switch (c) {
+      case 0: LaunchNoAlloc<0>(tt, cnt, ntiles[c], stage_len, g->d_params, s); break;
+      case 1: LaunchNoAlloc<1>(tt, cnt, ntiles[c], stage_len, g->d_params, s); break;
+      case 2: LaunchNoAlloc<2>(tt, cnt, ntiles[c], stage_len, g->d_params, s); break;
+      case 3: LaunchNoAlloc<3>(tt, cnt, ntiles[c], stage_len, g->d_params, s); break;
+      case 4: LaunchNoAlloc<4>(tt, cnt, ntiles[c], stage_len, g->d_params, s); break;
+      case 5: LaunchNoAlloc<5>(tt, cnt, ntiles[c], stage_len, g->d_params, s); break;
+      case 6: LaunchNoAlloc<6>(tt, cnt, ntiles[c], stage_len, g->d_params, s); break;
+      case 7: LaunchNoAlloc<7>(tt, cnt, ntiles[c], stage_len, g->d_params, s); break;
+      case 8: LaunchNoAlloc<8>(tt, cnt, ntiles[c], stage_len, g->d_params, s); break;
+      default: LaunchNoAlloc<9>(tt, cnt, ntiles[c], stage_len, g->d_params, s); break;
+    }
+    rc = CheckHip(hipGetLastError(), "no-allocation batch launch");
+  }
+  return rc;
+}
 
 int vg_batch_create(const vg_geometry *g, const vg_query_task *tasks, int32_t ntasks, vg_batch **out) {
   if (!g || !out || ntasks < 0 || (ntasks > 0 && !tasks)) return SetError(VG_ERR_INVALID, "null argument");
