@@ -362,6 +362,14 @@ typedef struct vg_nav_states {
 #define VG_NAV_ABBOX 0
 #define VG_NAV_SIMPLE 1
 #define VG_NAV_ABBOX_MIXED 2
+/* SimpleNavigator::FindNextBoundaryAndStep for baskets
+ * (navigation/SimpleNavigator.h:638-743): the basket's tracks share one
+ * current volume, taken as the top of the first track's state; the
+ * volume's DistanceToOut with pstep as step limit, DistanceToInMinimize over
+ * its daughters, push by 1e-9, no repair of steps <= 0 or kInfinity.  For
+ * vg_find_next_boundary_and_step only (the trace and serving loops step
+ * each track in its own volume). */
+#define VG_NAV_SIMPLE_BASKET 3
 
 /* Largest geometry depth the device navigator handles. */
 #define VG_NAV_MAX_DEPTH 16

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "base/Global.h"
+#include "base/SOA3D.h"
 #include "base/Transformation3D.h"
 #include "base/Vector3D.h"
 #include "management/GeoManager.h"
@@ -140,6 +141,32 @@ void vgref_nav_step(void *h, long n, const double *x, const double *y, const dou
   Activate(g);
   ABBoxNavigator abnav;
   SimpleNavigator snav;
+  if (kind == 3) {
+    // The basket interface (SimpleNavigator.h:638-743) over the whole basket.
+    SOA3D<Precision> gp(n), gd(n), lp(n), ld(n);
+    for (long i = 0; i < n; ++i) {
+      gp.set(i, x[i], y[i], z[i]);
+      gd.set(i, dx[i], dy[i], dz[i]);
+    }
+    std::vector<NavigationState *> cur(n), nxt(n);
+    for (long i = 0; i < n; ++i) {
+      cur[i] = Load(g, level_in[i], path_in, n, i);
+      nxt[i] = NavigationState::MakeInstance(g->depth);
+    }
+    std::vector<Precision> ps(n, pstep), safe(n), dist(n);
+    std::vector<int> work(n);
+    if (n > 0)
+      snav.FindNextBoundaryAndStep(gp, gd, lp, ld, cur.data(), nxt.data(), ps.data(), safe.data(), dist.data(),
+                                   work.data());
+    for (long i = 0; i < n; ++i) {
+      Store(g, nxt[i], depth, n, i, level_out, path_out);
+      boundary_out[i] = nxt[i]->IsOnBoundary() ? 1 : 0;
+      step_out[i] = dist[i];
+      NavigationState::ReleaseInstance(cur[i]);
+      NavigationState::ReleaseInstance(nxt[i]);
+    }
+    return;
+  }
   NavigationState *nst = NavigationState::MakeInstance(g->depth);
   for (long i = 0; i < n; ++i) {
     NavigationState *cur = Load(g, level_in[i], path_in, n, i);

@@ -262,6 +262,48 @@ static void FindNextBoundaryAndStep(const NavGeom &g, V3 gp, V3 gd, const NavSta
   }
 }
 
+// SimpleNavigator::FindNextBoundaryAndStep for baskets (kind 3),
+// navigation/SimpleNavigator.h:638-743, for one track of a basket whose
+// current volume is `vol` (the reference takes the top volume of the
+// basket's first state for every track, :672): local point and direction
+// from the track's own TopMatrix (:660-667); the volume's DistanceToOut with
+// the physics step as step limit, no repair of values <= 0 (:678-679;
+// ShapeImplementationHelper.h:677-692 fills -1 / -2, both "no daughter");
+// DistanceToInMinimize over the daughters in order: DistanceToIn with the
+// current distance as step limit, taken if strictly smaller
+// (ShapeImplementationHelper.h:637-658); distance > pstep -> physics step,
+// off boundary (:708-713); otherwise on boundary, push by 1e-9 (:721-723),
+// then LocatePoint into the daughter or RelocatePointFromPath (:725-740).
+static void FindNextBoundaryAndStepBasket(const NavGeom &g, V3 gp, V3 gd, const NavState &cur, NavState &nst,
+                                          double pstep, double &step, int vol) {
+  Transform m = TopMatrix(g, cur);
+  V3 lp = m.ApplyGeneric(gp);
+  V3 ld = m.ApplyDirectionGeneric(gd);
+  int next = -1;
+  step = g.ShapeOf(vol)->DistanceToOut(lp, ld, pstep);
+  for (int k = g.DBegin(vol); k < g.DEnd(vol); ++k) {
+    double dd = g.DistanceToIn(g.daughters[k], lp, ld, step);
+    if (dd < step) {
+      step = dd;
+      next = k - g.DBegin(vol);
+    }
+  }
+  nst = cur;
+  if (step > pstep) {
+    step = pstep;
+    nst.boundary = false;
+    return;
+  }
+  nst.boundary = true;
+  V3 np{ld.x * (step + 1e-9) + lp.x, ld.y * (step + 1e-9) + lp.y, ld.z * (step + 1e-9) + lp.z};
+  if (next > -1) {
+    int nv = g.daughters[g.DBegin(vol) + next];
+    LocatePoint(g, nv, g.xf[nv].ApplyGeneric(np), nst, false);
+    return;
+  }
+  Relocate(g, np, nst);
+}
+
 // ABBoxNavigator::GetSafety, ABBoxNavigator.h:428-452
 static double GetSafety(const NavGeom &g, V3 gp, const NavState &cur) {
   Transform m = TopMatrix(g, cur);
@@ -386,13 +428,20 @@ void vgo_nav_step(void *h, long n, const double *x, const double *y, const doubl
                   int *level_out, int *path_out, unsigned char *boundary_out, double *step_out, double pstep, int kind,
                   int nthreads) {
   auto *g = static_cast<NavGeom *>(h);
+  // Basket navigator: the current volume of the whole basket is the top of
+  // the first track's state.
+  const int basket_vol = (kind == 3 && n > 0 && level_in[0] > 0) ? path_in[(level_in[0] - 1) * n + 0] : -1;
   ParallelFor(n, nthreads, [&](long b, long e) {
     for (long i = b; i < e; ++i) {
       NavState cur, nst;
       cur.level = level_in[i];
       for (int l = 0; l < cur.level; ++l) cur.path[l] = path_in[l * n + i];
       double step;
-      FindNextBoundaryAndStep(*g, V3{x[i], y[i], z[i]}, V3{dx[i], dy[i], dz[i]}, cur, nst, pstep, step, kind);
+      if (kind == 3)
+        FindNextBoundaryAndStepBasket(*g, V3{x[i], y[i], z[i]}, V3{dx[i], dy[i], dz[i]}, cur, nst, pstep, step,
+                                      basket_vol);
+      else
+        FindNextBoundaryAndStep(*g, V3{x[i], y[i], z[i]}, V3{dx[i], dy[i], dz[i]}, cur, nst, pstep, step, kind);
       level_out[i] = nst.level;
       for (int l = 0; l < depth; ++l) path_out[l * n + i] = l < nst.level ? nst.path[l] : -1;
       boundary_out[i] = nst.boundary ? 1 : 0;

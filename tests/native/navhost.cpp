@@ -78,7 +78,15 @@ void Step(const HostGeom *h, long n, const double *x, const double *y, const dou
       DPath<D> c, o;
       Load(level_in, path_in, n, i, c);
       double step = 0;
-      if (c.level > 0 && kind == kExactOnlyKind) {
+      if (kind == VG_NAV_SIMPLE_BASKET) {
+        DPath<D> c0;
+        Load(level_in, path_in, n, 0, c0);
+        if (c.level > 0 && c0.level > 0)
+          FindNextBoundaryAndStepBasket(h->dev, V3{x[i], y[i], z[i]}, V3{dx[i], dy[i], dz[i]}, c, o, pstep, step,
+                                        c0.Top());
+        else
+          o = c;
+      } else if (c.level > 0 && kind == kExactOnlyKind) {
         FindNextBoundaryAndStep<D, true>(h->dev, V3{x[i], y[i], z[i]}, V3{dx[i], dy[i], dz[i]}, c, o, pstep, step,
                                          VG_NAV_ABBOX);
       } else if (c.level > 0 && kind == VG_NAV_ABBOX_MIXED) {

@@ -233,3 +233,39 @@ def test_gpu_nav_states_pool_c_abi(gpu, nested_dev):
     assert rc == -1 and "alias" in msg
     rc, _ = capi.status("vg_trace_tracks", geom.handle, 7, *[x.data_ptr()] * 6, 0, 1.0, 10, *[None] * 8)
     assert rc == -1
+
+
+def test_gpu_basket_navigator_vs_oracle(gpu, oracle, nested_dev):
+    """SimpleNavigator's basket step (navigator kind 3) on baskets of tracks
+    sharing a current volume, against the oracle (itself checked against the
+    reference's basket interface); also a basket of world states whose
+    points lie outside the world box, where it differs from the scalar
+    navigator."""
+    from test_oracle_nav import baskets_by_volume
+    _, flat, geom = nested_dev
+    rng = np.random.Generator(np.random.Philox(2991))
+    n = 6000
+    r = rng.uniform(0.0, 1000.0, n)
+    ph = rng.uniform(0, 2 * np.pi, n)
+    p = np.stack([r * np.cos(ph), r * np.sin(ph), rng.uniform(-1900, 1900, n)])
+    d = W.isotropic(n, rng)
+    nav = po.Nav(oracle, flat)
+    level, path = nav.locate(p)
+    baskets = [(p[:, i].copy(), d[:, i].copy(), level[i].copy(), path[:, i].copy())
+               for i in baskets_by_volume(level, path)]
+    m = 64
+    q = np.stack([rng.uniform(1001, 1100, m), rng.uniform(-900, 900, m), rng.uniform(-1900, 1900, m)])
+    lw = np.ones(m, np.int32)
+    pw = np.full((nav.depth, m), -1, np.int32)
+    pw[0] = flat.world
+    baskets.append((q, W.isotropic(m, rng), lw, pw))
+    for q, v, lo, pa in baskets:
+        s = N.NavStates.from_arrays(geom, lo, pa, device=gpu)
+        for pstep in (np.inf, 7.5):
+            olo, opa, obo, ost = nav.step(q, v, lo, pa, pstep=pstep, kind=3)
+            ps = torch.full((q.shape[1],), pstep, dtype=torch.float64, device=gpu)
+            new, step = N.find_next_boundary_and_step(geom, dev(q, gpu), dev(v, gpu), s, pstep=ps,
+                                                      navigator=N.SIMPLE_BASKET)
+            glo, gpa, gbo = states_host(new)
+            assert (glo == olo).all() and (gpa == opa).all() and (gbo == obo).all()
+            assert close(host(step), ost).all()

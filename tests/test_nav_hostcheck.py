@@ -74,3 +74,18 @@ def test_exact_walk_matches_oracle(oracle, hostnav, case):
         assert same(x, y).all()
     for x, y in zip(O.trace(p, d, kind=0, nthreads=8), H.trace(p, d, kind=9, nthreads=8)):
         assert same(x, y).all()
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_basket_navigator_host_build_matches_oracle(oracle, hostnav, case):
+    """The basket SimpleNavigator (kind 3) of the product code, on baskets of
+    tracks sharing a current volume, against the oracle."""
+    from test_oracle_nav import baskets_by_volume
+    flat, p, d = CASES[case]
+    O, H = po.Nav(oracle, flat), po.Nav(hostnav, flat)
+    level, path = O.locate(p, nthreads=8)
+    for idx in baskets_by_volume(level, path):
+        q, v, lo, pa = p[:, idx].copy(), d[:, idx].copy(), level[idx].copy(), path[:, idx].copy()
+        for pstep in (np.inf, 7.5):
+            for x, y in zip(O.step(q, v, lo, pa, pstep=pstep, kind=3), H.step(q, v, lo, pa, pstep=pstep, kind=3)):
+                assert same(x, y).all(), (case, pstep)

@@ -150,3 +150,91 @@ def test_locate_trd_centres(nested, nested_nav):
 def test_safety_at_origin(nested_nav):
     level, path = nested_nav.locate(np.zeros((3, 1)))
     assert nested_nav.safety(np.zeros((3, 1)), level, path)[0] == 100.0
+
+
+def baskets_by_volume(level, path, count=4):
+    """Indices of the tracks of the `count` most common current volumes: the
+    basket navigator assumes a basket shares one current volume."""
+    top = path[np.maximum(level - 1, 0), np.arange(level.size)]
+    top = np.where(level > 0, top, -1)
+    vols, counts = np.unique(top[top >= 0], return_counts=True)
+    order = np.argsort(-counts)[:count]
+    return [np.flatnonzero(top == v) for v in vols[order]]
+
+
+@pytest.mark.parametrize("geometry", ["nested", "zoo"])
+@pytest.mark.parametrize("pstep", [np.inf, 7.5])
+def test_basket_navigator_against_live_reference(oracle, reference, geometry, pstep):
+    """SimpleNavigator's basket FindNextBoundaryAndStep (navigator kind 3,
+    SimpleNavigator.h:638-743): the oracle against the reference's own basket
+    interface on baskets of tracks sharing a current volume."""
+    if geometry == "nested":
+        _, flat = W.nested_geometry()
+        rng = np.random.Generator(np.random.Philox(1991))
+        n = 4000
+        r = rng.uniform(0.0, 1000.0, n)
+        ph = rng.uniform(0, 2 * np.pi, n)
+        p = np.stack([r * np.cos(ph), r * np.sin(ph), rng.uniform(-1900, 1900, n)])
+        d = W.isotropic(n, rng)
+    else:
+        _, flat, placed = W.nav_zoo_geometry(names=ZOO_REF_NAMES)
+        centres = [flat.placed_transform[12 * i:12 * i + 3] for _, i in placed]
+        p, d = W.nav_zoo_tracks(4000, centres, seed=1992)
+    a, b = po.Nav(oracle, flat), po.Nav(reference, flat)
+    level, path = a.locate(p)
+    for idx in baskets_by_volume(level, path):
+        q, v = p[:, idx].copy(), d[:, idx].copy()
+        lo, pa = level[idx].copy(), path[:, idx].copy()
+        got = a.step(q, v, lo, pa, pstep=pstep, kind=3)
+        want = b.step(q, v, lo, pa, pstep=pstep, kind=3)
+        for x, y in zip(got, want):
+            assert same(x, y).all(), (geometry, len(idx))
+        assert (got[3] <= pstep).all()
+
+
+def test_basket_navigator_differs_from_scalar_where_the_reference_does(oracle, reference):
+    """States in the world whose points lie outside the world box: the
+    world's DistanceToOut is negative there, which the scalar SimpleNavigator
+    repairs (step <= 0 -> kInfinity -> leave the world with step 0) and the
+    basket navigator does not (it steps by the negative distance, pushes by
+    1e-9 and relocates).  Oracle and reference agree on both, and the two
+    navigators differ wherever that distance is negative."""
+    _, flat = W.nested_geometry()
+    rng = np.random.Generator(np.random.Philox(77))
+    n = 64
+    p = np.stack([rng.uniform(1001, 1100, n), rng.uniform(-900, 900, n), rng.uniform(-1900, 1900, n)])
+    d = W.isotropic(n, rng)
+    world = flat.world
+    level = np.ones(n, np.int32)
+    depth = po.Nav(oracle, flat).depth
+    path = np.full((depth, n), -1, np.int32)
+    path[0] = world
+    a, b = po.Nav(oracle, flat), po.Nav(reference, flat)
+    ob, rb = a.step(p, d, level, path, kind=3), b.step(p, d, level, path, kind=3)
+    for x, y in zip(ob, rb):
+        assert same(x, y).all()
+    os_, rs = a.step(p, d, level, path, kind=1), b.step(p, d, level, path, kind=1)
+    for x, y in zip(os_, rs):
+        assert same(x, y).all()
+    neg = ob[3] < 0  # the world's DistanceToOut is negative on these tracks
+    assert neg.sum() > 8
+    assert (os_[3][neg] == 0).all() and (os_[0][neg] == 0).all()  # scalar: step 0, out of the world
+
+
+def test_basket_navigator_closed_form(nested, nested_nav):
+    """A basket of tracks in the world at radii below the innermost shell,
+    moving radially outwards: each steps to the shell's inner radius and is
+    located in it."""
+    gm, _ = nested
+    sh = shells(gm)
+    rmin0 = 100.0
+    n = 16
+    ph = np.linspace(0, 2 * np.pi, n, endpoint=False)
+    r0 = np.linspace(1.0, 90.0, n)
+    p = np.stack([r0 * np.cos(ph), r0 * np.sin(ph), np.linspace(-500, 500, n)])
+    d = np.stack([np.cos(ph), np.sin(ph), np.zeros(n)])
+    level, path = nested_nav.locate(p)
+    assert (level == 1).all()
+    lo, pa, bo, st = nested_nav.step(p, d, level, path, kind=3)
+    np.testing.assert_allclose(st, rmin0 - r0, rtol=0, atol=1e-9)
+    assert (bo == 1).all() and (lo == 2).all() and (pa[1] == sh[0]).all()

@@ -111,7 +111,18 @@ __global__ __launch_bounds__(kNavBlock) void StepKernel(const NavDev g, const St
   LoadState(cur, i, c);
   double step = 0;
   const double ps = a.pstep ? a.pstep[i] : kInfinity;
-  if (ValidState(g, c)) {
+  if (a.kind == VG_NAV_SIMPLE_BASKET) {
+    // The basket's current volume: the top of the first track's state.
+    DPath<D> c0;
+    LoadState(cur, 0, c0);
+    if (ValidState(g, c0) && ValidState(g, c)) {
+      FindNextBoundaryAndStepBasket<D, M>(g, V3{a.x[i], a.y[i], a.z[i]}, V3{a.dx[i], a.dy[i], a.dz[i]}, c, o, ps,
+                                          step, c0.Top());
+    } else {
+      o = c;
+      if (c.level != 0) step = __builtin_nan("");
+    }
+  } else if (ValidState(g, c)) {
     FindNextBoundaryAndStep<D, false, M, MX>(g, V3{a.x[i], a.y[i], a.z[i]}, V3{a.dx[i], a.dy[i], a.dz[i]}, c, o, ps,
                                          step, a.kind);
   } else {

@@ -12,9 +12,13 @@ using namespace vga;
 
 namespace {
 
-int CheckNav(const vg_geometry *g, int kind) {
+int CheckNav(const vg_geometry *g, int kind, bool basket_ok = false) {
   if (!g) return SetError(VG_ERR_INVALID, "null geometry");
-  if (kind != VG_NAV_ABBOX && kind != VG_NAV_SIMPLE && kind != VG_NAV_ABBOX_MIXED) return SetError(VG_ERR_INVALID, "unknown navigator");
+  if (kind != VG_NAV_ABBOX && kind != VG_NAV_SIMPLE && kind != VG_NAV_ABBOX_MIXED &&
+      !(basket_ok && kind == VG_NAV_SIMPLE_BASKET))
+    return SetError(VG_ERR_INVALID, kind == VG_NAV_SIMPLE_BASKET ? "the basket navigator steps baskets only "
+                                                                   "(vg_find_next_boundary_and_step)"
+                                                                 : "unknown navigator");
   if (g->max_depth > VG_NAV_MAX_DEPTH)
     return SetError(VG_ERR_UNSUPPORTED, "geometry depth " + std::to_string(g->max_depth) + " exceeds " +
                                             std::to_string(VG_NAV_MAX_DEPTH));
@@ -125,7 +129,7 @@ int vg_find_next_boundary_and_step(const vg_geometry *g, int32_t navigator, cons
                                    const double *z, const double *dx, const double *dy, const double *dz,
                                    const double *pstep, size_t n, const vg_nav_states *cur, vg_nav_states *next,
                                    double *step, void *stream) {
-  int rc = CheckNav(g, navigator);
+  int rc = CheckNav(g, navigator, true);
   if (!rc) rc = CheckStates(g, cur, n, "current states");
   if (!rc) rc = CheckStates(g, next, n, "new states");
   if (rc) return rc;

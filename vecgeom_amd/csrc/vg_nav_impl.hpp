@@ -649,6 +649,51 @@ VGA_NAV int FindNextBoundaryAndStep(const NavDev &g, V3 gp, V3 gd, const DPath<D
   return exact;
 }
 
+// SimpleNavigator::FindNextBoundaryAndStep for baskets
+// (navigation/SimpleNavigator.h:638-743), for one track of a basket whose
+// current volume is `vol` (the top of the basket's first state, :672): the
+// track's own TopMatrix (:660-667); the volume's DistanceToOut with pstep as
+// step limit and no repair of values <= 0 (:678-679); DistanceToInMinimize
+// over the daughters in order, each with the current distance as step
+// limit, taken if strictly smaller (ShapeImplementationHelper.h:637-658);
+// distance > pstep -> physics step off boundary (:708-713); otherwise push
+// by 1e-9 and LocatePoint into the daughter or RelocatePointFromPath
+// (:715-740).
+template <int D, uint32_t M = kAllShapes>
+VGA_NAV void FindNextBoundaryAndStepBasket(const NavDev &g, V3 gp, V3 gd, const DPath<D> &cur, DPath<D> &nst,
+                                           double pstep, double &step, int vol) {
+  const Xf m = TopMatrix(g, cur);
+  const V3 lp = ApplyGeneric(m, gp);
+  const V3 ld = Rotate(m, gd);
+  const PlacedRec &C = g.Pv(vol);
+  step = DDistOut<M>(C.type, g.params + C.param_off, lp, ld, pstep);
+  int next = -1;
+  for (int k = C.dbegin; k < C.dend; ++k) {
+    const double dd = PlacedDistIn<M>(g, g.Daughter(k), lp, ld, step);
+    if (dd < step) {
+      step = dd;
+      next = k - C.dbegin;
+    }
+  }
+  nst = cur;
+  if (step > pstep) {
+    step = pstep;
+    nst.boundary = 0;
+    return;
+  }
+  nst.boundary = 1;
+  const double push = step + 1E-9;
+  const V3 np{ld.x * push + lp.x, ld.y * push + lp.y, ld.z * push + lp.z};
+  if (next > -1) {
+    const int nv = g.Daughter(C.dbegin + next);
+    Xf x;
+    LoadXf(g.Pv(nv), x);
+    LocatePoint<D, M>(g, nv, ApplyGeneric(x, np), nst, false);
+    return;
+  }
+  Relocate<D, M>(g, np, nst);
+}
+
 template <int D, uint32_t M = kAllShapes>
 VGA_NAV double GetSafety(const NavDev &g, V3 gp, const DPath<D> &cur) {
   const Xf m = TopMatrix(g, cur);

@@ -25,6 +25,7 @@ from . import capi
 ABBOX = 0
 SIMPLE = 1
 ABBOX_MIXED = 2  # ABBox navigator with the FP32 daughter filter (same results)
+SIMPLE_BASKET = 3  # SimpleNavigator's basket step: one current volume for the whole basket
 MAX_DEPTH = 16
 HASH_SEED = 1469598103934665603
 HASH_MUL = 1099511628211
