@@ -164,9 +164,24 @@ int vg_geometry_placed_aabb(const vg_geometry *geometry, int32_t placed, double 
  * length n; n may be any value >= 0.  step_max may be NULL (= +inf, the
  * reference's default argument). */
 
+/* VPlacedVolume::Contains(point, localPoint&) (volumes/PlacedVolume.h:153-156)
+ * in basket form: out[i] as Contains, and the point transformed to the
+ * volume's local frame in lx, ly, lz (device arrays; each nullable). */
+int vg_contains_local(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z,
+                      size_t n, uint8_t *out, double *lx, double *ly, double *lz, void *stream);
+/* VPlacedVolume::UnplacedContains(localPoint) (volumes/PlacedVolume.h:158-160)
+ * in basket form: the unplaced shape's Contains of local points. */
+int vg_unplaced_contains(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z,
+                         size_t n, uint8_t *out, void *stream);
+
 /* VPlacedVolume::Contains(SOA3D, bool*)  (volumes/PlacedVolume.h:147) */
 int vg_contains(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z, size_t n,
                 uint8_t *out, void *stream);
+
+/* EInside codes of Inside (base/Global.h:198-205). */
+#define VG_INSIDE_INSIDE 0
+#define VG_INSIDE_SURFACE 1
+#define VG_INSIDE_OUTSIDE 2
 
 /* VPlacedVolume::Inside(SOA3D, Inside_t*)  (volumes/PlacedVolume.h:165) */
 int vg_inside(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z, size_t n,
@@ -227,6 +242,19 @@ int vg_safety_to_out_minimize(const vg_geometry *g, int32_t placed, const double
 #define VG_Q_PLACED_DISTANCE_TO_OUT 4 /* out: double (PlacedDistanceToOut)           */
 #define VG_Q_SAFETY_TO_IN 5           /* out: double (SafetyToIn, mother frame)      */
 #define VG_Q_SAFETY_TO_OUT 6          /* out: double (SafetyToOut, local frame)      */
+/* The scalar VPlacedVolume virtuals (volumes/PlacedVolume.h:144-250):
+ * one point (and direction) in host memory, evaluated on the device by the
+ * basket kernels, synchronously (the call returns with the result).
+ * query: a VG_Q_* code above, or VG_Q_CONTAINS_LOCAL (Contains(point,
+ * localPoint&): local receives the local point) or VG_Q_UNPLACED_CONTAINS
+ * (point is local).  *result: Contains 0/1, Inside the EInside code,
+ * otherwise the distance or safety.  dir is read by the distance queries
+ * only.  Calls on one geometry are serialised. */
+#define VG_Q_CONTAINS_LOCAL 7
+#define VG_Q_UNPLACED_CONTAINS 8
+int vg_scalar_query(vg_geometry *g, int32_t placed, int32_t query, const double point[3], const double dir[3],
+                    double step_max, double *result, double local[3]);
+
 /* Input element types: VG_F64 = double arrays (the SOA3D<double> of the
  * reference); VG_F32 = float arrays (FP32 storage: every value is promoted to
  * double before any arithmetic, so results equal the FP64 form on the same

@@ -265,3 +265,39 @@ def test_gpu_extra_variants_vs_oracle(gpu, oracle, name):
     assert_agree(family, "dist_in", got["dist_in"], o.query(po.Q_DIST_IN, g["p"], g["d"], g["sm"]), name)
     assert_agree(family, "dist_out", got["dist_out"], o.query(po.Q_DIST_OUT, g["lp"], g["ld"], g["sm"]), name)
     assert_agree(family, "safe_out", got["safe_out"], o.query(po.Q_SAFE_OUT, g["lp"]), name)
+
+
+def test_gpu_contains_local_unplaced_contains_and_scalar(gpu, oracle):
+    """Contains(point, localPoint&) and UnplacedContains in basket form on
+    every zoo shape under a random placement, against the oracle (Contains
+    of the placed shape; the local point is the placement's Transform), and
+    the scalar entry point on a sample of the same points against the
+    basket results of every query."""
+    from vecgeom_amd import batch as B
+    import pyoracle as po
+    shapes = [(n, u) for n, u in W.zoo_shapes() if Q.capi.lib().vg_shape_supported(u.type)]
+    rng = np.random.Generator(np.random.Philox(4711))
+    xfs = [W.random_placement(rng) for _ in shapes]
+    geom, ids, _ = G.single_shape_geometry([u for _, u in shapes], xfs)
+    for (name, u), xf, pid in zip(shapes, xfs, ids):
+        inp = W.shape_inputs(geom.placed_extent(pid), xf, 5000, seed=pid + 4712)
+        p = tuple(torch.from_numpy(np.ascontiguousarray(inp["p"][i])).to(gpu) for i in range(3))
+        c, loc = Q.contains_local(geom, pid, p)
+        want = oracle.placed(u.type, u.params, xf.array()).query(po.Q_CONTAINS, inp["p"])
+        assert (c.cpu().numpy() == want).all(), name
+        got_loc = np.stack([a.cpu().numpy() for a in loc])
+        assert (got_loc == inp["lp"]).all(), name  # the reference's operation order, bit for bit
+        uc = Q.unplaced_contains(geom, pid, loc)
+        assert (uc.cpu().numpy() == want).all(), name
+        dirs = tuple(torch.from_numpy(np.ascontiguousarray(inp["d"][i])).to(gpu) for i in range(3))
+        basket = {B.CONTAINS: Q.contains(geom, pid, p).cpu().numpy(),
+                  B.INSIDE: Q.inside(geom, pid, p).cpu().numpy(),
+                  B.DISTANCE_TO_IN: Q.distance_to_in(geom, pid, p, dirs).cpu().numpy(),
+                  B.SAFETY_TO_IN: Q.safety_to_in(geom, pid, p).cpu().numpy()}
+        for i in range(0, 5000, 997):
+            for q, arr in basket.items():
+                r = Q.scalar(geom, pid, q, inp["p"][:, i], inp["d"][:, i])
+                assert (r == float(arr[i])) or (np.isnan(r) and np.isnan(arr[i])), (name, q, i)
+            r, lp = Q.scalar(geom, pid, Q.CONTAINS_LOCAL, inp["p"][:, i])
+            assert r == float(want[i]) and lp == tuple(inp["lp"][:, i]), (name, i)
+    geom.free()

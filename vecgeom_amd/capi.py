@@ -61,6 +61,9 @@ _SIGS = {
     "vg_safety_to_in_minimize": [c_vp, c_i32, c_vp, c_vp, c_vp, c_sz, c_vp, c_vp],
     "vg_safety_to_out": [c_vp, c_i32, c_vp, c_vp, c_vp, c_sz, c_vp, c_vp],
     "vg_safety_to_out_minimize": [c_vp, c_i32, c_vp, c_vp, c_vp, c_sz, c_vp, c_vp],
+    "vg_contains_local": [c_vp, c_i32, c_vp, c_vp, c_vp, c_sz, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "vg_unplaced_contains": [c_vp, c_i32, c_vp, c_vp, c_vp, c_sz, c_vp, c_vp],
+    "vg_scalar_query": [c_vp, c_i32, c_i32, c_vp, c_vp, ctypes.c_double, c_vp, c_vp],
 }
 _RESTYPE = {"vg_last_error": ctypes.c_char_p, "vg_version": ctypes.c_char_p}
 # Entry points added by later layers register here (navigator, batching,

@@ -37,7 +37,9 @@ void FreeDevice(vg_geometry *g) {
   if (g->d_b32side) (void)hipFree(g->d_b32side);
   if (g->d_extent32) (void)hipFree(g->d_extent32);
   if (g->d_slot_aabb) (void)hipFree(g->d_slot_aabb);
+  if (g->d_scalar) (void)hipFree(g->d_scalar);
   g->d_slot_aabb = nullptr;
+  g->d_scalar = nullptr;
   g->d_b32 = nullptr;
   g->d_b32side = nullptr;
   g->d_extent32 = nullptr;
@@ -114,6 +116,7 @@ int vg_geometry_upload(const vg_flat_geometry *f, vg_geometry **out) {
   if (!rc) rc = Upload(g->b32side, &g->d_b32side);
   if (!rc) rc = Upload(g->extent32, &g->d_extent32);
   if (!rc) rc = Upload(g->slot_aabb, &g->d_slot_aabb);
+  if (!rc) rc = Upload(std::vector<double>(16, 0.), &g->d_scalar);
   if (rc) {
     FreeDevice(g);
     delete g;
@@ -177,6 +180,89 @@ static QArgs Args(const double *x, const double *y, const double *z, const doubl
   a.pad = 0;
   a.n = n;
   return a;
+}
+
+int vg_contains_local(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z,
+                      size_t n, uint8_t *out, double *lx, double *ly, double *lz, void *stream) {
+  if (!ValidPlaced(g, placed)) return SetError(VG_ERR_INVALID, "bad geometry handle or placed id");
+  if (n == 0) return VG_OK;
+  if (!x || !y || !z || !out) return SetError(VG_ERR_INVALID, "null point or output array");
+  if (!ShapeHasKernels(g->placed[placed].type)) return SetError(VG_ERR_UNSUPPORTED, "no device kernels for the shape");
+  int rc = EnsureDevice();
+  if (rc) return rc;
+  return LaunchContainsExtra(g, placed, x, y, z, n, out, lx, ly, lz, false, static_cast<hipStream_t>(stream));
+}
+
+int vg_unplaced_contains(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z,
+                         size_t n, uint8_t *out, void *stream) {
+  if (!ValidPlaced(g, placed)) return SetError(VG_ERR_INVALID, "bad geometry handle or placed id");
+  if (n == 0) return VG_OK;
+  if (!x || !y || !z || !out) return SetError(VG_ERR_INVALID, "null point or output array");
+  if (!ShapeHasKernels(g->placed[placed].type)) return SetError(VG_ERR_UNSUPPORTED, "no device kernels for the shape");
+  int rc = EnsureDevice();
+  if (rc) return rc;
+  return LaunchContainsExtra(g, placed, x, y, z, n, out, nullptr, nullptr, nullptr, true,
+                             static_cast<hipStream_t>(stream));
+}
+
+int vg_scalar_query(vg_geometry *g, int32_t placed, int32_t query, const double point[3], const double dir[3],
+                    double step_max, double *result, double local[3]) {
+  if (!ValidPlaced(g, placed)) return SetError(VG_ERR_INVALID, "bad geometry handle or placed id");
+  if (!point || !result) return SetError(VG_ERR_INVALID, "null point or result");
+  const bool directional =
+      query == VG_Q_DISTANCE_TO_IN || query == VG_Q_DISTANCE_TO_OUT || query == VG_Q_PLACED_DISTANCE_TO_OUT;
+  if (directional && !dir) return SetError(VG_ERR_INVALID, "null direction");
+  if (query == VG_Q_CONTAINS_LOCAL && !local) return SetError(VG_ERR_INVALID, "null local point");
+  if (query < VG_Q_CONTAINS || query > VG_Q_UNPLACED_CONTAINS) return SetError(VG_ERR_INVALID, "unknown query");
+  int rc = EnsureDevice();
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lock(g->scalar_mutex);
+  double h[16] = {};
+  for (int i = 0; i < 3; ++i) h[i] = point[i];
+  if (directional)
+    for (int i = 0; i < 3; ++i) h[3 + i] = dir[i];
+  h[6] = step_max;
+  double *d = g->d_scalar;
+  if ((rc = CheckHip(hipMemcpy(d, h, 7 * sizeof(double), hipMemcpyHostToDevice), "scalar query upload"))) return rc;
+  void *o = d + 8;
+  switch (query) {
+    case VG_Q_CONTAINS: rc = vg_contains(g, placed, d, d + 1, d + 2, 1, static_cast<uint8_t *>(o), nullptr); break;
+    case VG_Q_INSIDE: rc = vg_inside(g, placed, d, d + 1, d + 2, 1, static_cast<int32_t *>(o), nullptr); break;
+    case VG_Q_DISTANCE_TO_IN:
+      rc = vg_distance_to_in(g, placed, d, d + 1, d + 2, d + 3, d + 4, d + 5, d + 6, 1, d + 8, nullptr);
+      break;
+    case VG_Q_DISTANCE_TO_OUT:
+      rc = vg_distance_to_out(g, placed, d, d + 1, d + 2, d + 3, d + 4, d + 5, d + 6, 1, d + 8, nullptr, nullptr);
+      break;
+    case VG_Q_PLACED_DISTANCE_TO_OUT:
+      rc = vg_placed_distance_to_out(g, placed, d, d + 1, d + 2, d + 3, d + 4, d + 5, d + 6, 1, d + 8, nullptr);
+      break;
+    case VG_Q_SAFETY_TO_IN: rc = vg_safety_to_in(g, placed, d, d + 1, d + 2, 1, d + 8, nullptr); break;
+    case VG_Q_SAFETY_TO_OUT: rc = vg_safety_to_out(g, placed, d, d + 1, d + 2, 1, d + 8, nullptr); break;
+    case VG_Q_CONTAINS_LOCAL:
+      rc = vg_contains_local(g, placed, d, d + 1, d + 2, 1, static_cast<uint8_t *>(o), d + 9, d + 10, d + 11, nullptr);
+      break;
+    default:  // VG_Q_UNPLACED_CONTAINS
+      rc = vg_unplaced_contains(g, placed, d, d + 1, d + 2, 1, static_cast<uint8_t *>(o), nullptr);
+      break;
+  }
+  if (rc) return rc;
+  if ((rc = CheckHip(hipMemcpy(h + 8, d + 8, 4 * sizeof(double), hipMemcpyDeviceToHost), "scalar query result")))
+    return rc;
+  if (query == VG_Q_CONTAINS || query == VG_Q_CONTAINS_LOCAL || query == VG_Q_UNPLACED_CONTAINS) {
+    uint8_t c;
+    std::memcpy(&c, h + 8, 1);
+    *result = c ? 1. : 0.;
+  } else if (query == VG_Q_INSIDE) {
+    int32_t v;
+    std::memcpy(&v, h + 8, 4);
+    *result = (double)v;
+  } else {
+    *result = h[8];
+  }
+  if (query == VG_Q_CONTAINS_LOCAL)
+    for (int i = 0; i < 3; ++i) local[i] = h[9 + i];
+  return VG_OK;
 }
 
 int vg_contains(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z, size_t n,

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -65,6 +66,8 @@ struct vg_geometry {
   float *d_b32side = nullptr;
   float *d_extent32 = nullptr;
   float *d_slot_aabb = nullptr;
+  double *d_scalar = nullptr;  // device staging of vg_scalar_query (16 doubles)
+  std::mutex scalar_mutex;     // one vg_scalar_query at a time per geometry
   int32_t nav_bad_type = 0;  // first shape type in the tree without device kernels (0: none)
   uint32_t type_mask = 0;    // bit t set: some placed volume has shape type t
 };
@@ -114,6 +117,10 @@ struct QArgs {
   int32_t pad;
   uint64_t n;
 };
+
+// Contains(point, localPoint&) / UnplacedContains baskets (vg_extra.hip).
+int LaunchContainsExtra(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z,
+                        uint64_t n, uint8_t *out, double *lx, double *ly, double *lz, bool local_in, hipStream_t s);
 
 // Launches one basket query on placed volume `placed` (vg_shape_kernels.hip).
 int LaunchQuery(const vg_geometry *g, int32_t placed, int q, const QArgs &a, hipStream_t s);

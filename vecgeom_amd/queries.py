@@ -59,6 +59,44 @@ def contains(geom, placed, p, out=None):
     return out
 
 
+def contains_local(geom, placed, p, out=None):
+    """Contains(point, localPoint&) (volumes/PlacedVolume.h:153-156):
+    returns (uint8 contains, (lx, ly, lz) local points)."""
+    n = _check_soa(p, "points")
+    out = _out(out, torch.uint8, n, p[0].device)
+    loc = tuple(torch.empty(n, dtype=torch.float64, device=p[0].device) for _ in range(3))
+    capi.call("vg_contains_local", geom.handle, int(placed), *_ptrs(p), n, out.data_ptr(), *_ptrs(loc), _stream())
+    return out, loc
+
+
+def unplaced_contains(geom, placed, lp, out=None):
+    """UnplacedContains(localPoint) (volumes/PlacedVolume.h:158-160)."""
+    n = _check_soa(lp, "local points")
+    out = _out(out, torch.uint8, n, lp[0].device)
+    capi.call("vg_unplaced_contains", geom.handle, int(placed), *_ptrs(lp), n, out.data_ptr(), _stream())
+    return out
+
+
+CONTAINS_LOCAL, UNPLACED_CONTAINS = 7, 8
+
+
+def scalar(geom, placed, query, point, direction=None, step_max=float("inf")):
+    """One scalar VPlacedVolume virtual on the device, synchronously
+    (vg_scalar_query): returns the result (and the local point for
+    CONTAINS_LOCAL).  query: the batch.* query codes, CONTAINS_LOCAL or
+    UNPLACED_CONTAINS."""
+    import ctypes
+    import numpy as np
+    p = np.ascontiguousarray(point, dtype=np.float64)
+    d = np.ascontiguousarray(direction if direction is not None else (0.0, 0.0, 0.0), dtype=np.float64)
+    r = ctypes.c_double()
+    loc = np.zeros(3)
+    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    capi.call("vg_scalar_query", geom.handle, int(placed), int(query), ptr(p), ptr(d), float(step_max),
+              ctypes.byref(r), ptr(loc))
+    return (r.value, tuple(loc)) if query == CONTAINS_LOCAL else r.value
+
+
 def inside(geom, placed, p, out=None):
     n = _check_soa(p, "points")
     out = _out(out, torch.int32, n, p[0].device)
