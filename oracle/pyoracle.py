@@ -144,6 +144,98 @@ def reference(fast=False):
     return _cache[key]
 
 
+def _truth():
+    key = ("truth",)
+    if key not in _cache:
+        path = os.path.join(HERE, "lib", "libvgotruth.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run __graft_entry__.build()")
+        lib = ctypes.CDLL(path)
+        lib.vgo_torus_window.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_long] + [ctypes.c_void_p] * 9
+        lib.vgo_torus_on_surface.argtypes = [ctypes.c_void_p, ctypes.c_long] + [ctypes.c_void_p] * 9
+        _cache[key] = lib
+    return _cache[key]
+
+
+def _rays(params, p, v):
+    par = np.ascontiguousarray(np.asarray(params, np.float64)[:5])
+    p = [np.ascontiguousarray(np.asarray(p[i], np.float64)) for i in range(3)]
+    v = [np.ascontiguousarray(np.asarray(v[i], np.float64)) for i in range(3)]
+    return par, p, v
+
+
+def torus_window(params, p, v, out, tol=1e-9):
+    """Binary128 window of acceptable first-crossing distances of rays
+    (p, v) (local frame, 3 x n) for the torus (rmin, rmax, rtor, sphi, dphi),
+    DistanceToIn (out False) or DistanceToOut (out True): (lo, hi) per ray,
+    NaN where the start is not one the query is defined for or lies within
+    tol (a scalar or one per ray) of the surface (vgo_truth.cpp)."""
+    par, p, v = _rays(params, p, v)
+    n = len(p[0])
+    tol = np.ascontiguousarray(np.broadcast_to(np.asarray(tol, np.float64), (n,)))
+    lo, hi = np.empty(n), np.empty(n)
+    _truth().vgo_torus_window(_ptr(par), int(bool(out)), n, *[_ptr(a) for a in p], *[_ptr(a) for a in v],
+                              _ptr(tol), _ptr(lo), _ptr(hi))
+    return lo, hi
+
+
+def torus_on_surface(params, p, v, t, tol=1e-9):
+    """Per ray: the point at the finite distance t lies within tol of a
+    boundary surface of the torus (binary128, vgo_truth.cpp)."""
+    par, p, v = _rays(params, p, v)
+    t = np.ascontiguousarray(np.asarray(t, np.float64))
+    tol = np.ascontiguousarray(np.broadcast_to(np.asarray(tol, np.float64), t.shape))
+    on = np.empty(len(t), np.uint8)
+    _truth().vgo_torus_on_surface(_ptr(par), len(t), *[_ptr(a) for a in p], *[_ptr(a) for a in v], _ptr(t),
+                                  _ptr(tol), _ptr(on))
+    return on.astype(bool)
+
+
+def torus_distance_ok(params, p, v, got, out, tol=1e-9):
+    """Per lane: the Torus distance `got` of the local ray (p, v) is right to
+    within the surface tolerance.  Where the query is defined (a start
+    outside for DistanceToIn, out False; inside for DistanceToOut) it lies
+    in the window of the exact first boundary crossing, widened by the FP64
+    bar max(1e-9, 1e-12 |got|), and is +inf only where the ray never gets tol
+    deep into (or out of) the solid.  Elsewhere -- starts on the surface,
+    and starts the query is not defined for, whose answer is an artefact of
+    the algorithm -- it is a finite distance to a point within tol + bar of
+    a boundary surface, or +inf for a start off the surface whose ray never
+    comes within tol of the boundary.
+
+    The reference's quartic takes the direction as a unit vector; a
+    direction of norm 1 + e (float32 inputs promoted to double: e ~ 1e-8)
+    moves its roots by up to ~e times the torus' size.  The crossings are
+    judged along v/|v|, and a lane passes with the tolerance tol or with
+    tol widened by 4 e (rtor + rmax + |p|)."""
+    got = np.asarray(got, dtype=np.float64)
+    p = np.asarray(p, np.float64)
+    v = np.asarray(v, np.float64)
+    nv = np.sqrt((v * v).sum(axis=0))
+    vh = v / nv
+    e = np.abs(nv * nv - 1.0)
+    wide = tol + 4.0 * e * (float(params[2]) + float(params[1]) + np.sqrt((p * p).sum(axis=0)))
+    fin_got = np.isfinite(got)
+    bar = np.maximum(1e-9, 1e-12 * np.where(fin_got, np.abs(got), 0.0))
+
+    def check(t):
+        lo, hi = torus_window(params, p, vh, out, t)
+        with np.errstate(invalid="ignore"):
+            inside = (got >= lo - bar) & (got <= hi + bar)
+        defined = np.where(np.isinf(got), np.isinf(hi), inside)
+        # Starts the query is not defined for: +inf only if the ray never
+        # comes within t of the boundary (the other query's window opens
+        # nowhere), a finite distance only onto a boundary surface.
+        lo2, _ = torus_window(params, p, vh, not out, t)
+        undefined = np.where(np.isinf(got), np.isinf(lo2), torus_on_surface(params, p, vh, got, t + bar))
+        return np.where(np.isnan(lo), undefined, defined)
+
+    ok = check(np.full(got.shape, float(tol)))
+    if not ok.all() and (wide > tol).any():
+        ok |= check(wide)
+    return ok
+
+
 # ---------------------------------------------------------------- navigation
 _ip = ctypes.POINTER(ctypes.c_int)
 

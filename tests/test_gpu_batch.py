@@ -10,7 +10,7 @@ import pytest
 import torch
 
 import pyoracle as po
-from conftest import assert_agree, same
+from conftest import assert_agree, local_rays, same
 from vecgeom_amd import batch as B
 from vecgeom_amd import capi
 from vecgeom_amd import geometry as G
@@ -124,7 +124,8 @@ def test_fp32_storage_matches_oracle_on_promoted_inputs(gpu, zoo, oracle):
         o = oracle.placed(u.type, u.params, xf.array())
         args = (pf,) if df is None else (pf, df, np.full(t.n, 1e30))
         want = o.query(ORACLE_Q[q], *args)
-        assert_agree(name, NAMES[q], t.out.cpu().numpy(), want, "fp32 storage")
+        tor = (u.params, *local_rays(xf, pf, df, q in LOCAL)) if df is not None else None
+        assert_agree(name, NAMES[q], t.out.cpu().numpy(), want, "fp32 storage", torus=tor)
     b.free()
 
 

@@ -19,7 +19,7 @@ import pytest
 import torch
 
 import pyoracle as po
-from conftest import assert_agree
+from conftest import assert_agree, local_rays
 from vecgeom_amd import batch as B
 from vecgeom_amd import capi
 from vecgeom_amd import geometry as G
@@ -87,11 +87,16 @@ def make_task(pid, q, inp, device, mixed, count=None):
                   fp64_count=count)
 
 
-def oracle_out(oracle, u, xf, q, inp):
-    o = oracle.placed(u.type, u.params, xf.array())
-    # The float inputs promoted to double.
+def promoted(q, inp):
+    """The float inputs of query q promoted to double."""
     p = inp["lp32"].astype(np.float64) if q in LOCAL else inp["p"]
     d = inp["ld32"].astype(np.float64) if q in LOCAL else inp["d"]
+    return p, d
+
+
+def oracle_out(oracle, u, xf, q, inp):
+    o = oracle.placed(u.type, u.params, xf.array())
+    p, d = promoted(q, inp)
     sm = np.full(p.shape[1], 1e30) if q in B.DIRECTIONAL else None
     return o.query(ORACLE_Q[q], p, d if q in B.DIRECTIONAL else None, sm, nthreads=8)
 
@@ -108,7 +113,8 @@ def test_mixed_form_vs_oracle_and_fp32_storage(gpu, oracle, mixed_zoo, k):
         got = B.query(geom, make_task(ids[k], q, inp, gpu, True, count)).cpu().numpy()
         ref = B.query(geom, make_task(ids[k], q, inp, gpu, False)).cpu().numpy()
         assert same_lanes(got, ref).all(), f"{name} {NAMES[q]}: mixed differs from FP32 storage"
-        assert_agree(name, NAMES[q], got, oracle_out(oracle, u, xfs[k], q, inp), "mixed vs oracle")
+        tor = (u.params, *local_rays(xfs[k], *promoted(q, inp), q in LOCAL)) if q in B.DIRECTIONAL else None
+        assert_agree(name, NAMES[q], got, oracle_out(oracle, u, xfs[k], q, inp), "mixed vs oracle", torus=tor)
         n64 = int(count.item())
         assert 0 <= n64 <= N
         if q in (B.CONTAINS, B.INSIDE):

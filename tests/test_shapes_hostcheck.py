@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import pyoracle as po
-from conftest import ROOT, golden_cases, load_golden, same
+from conftest import ROOT, assert_agree, golden_cases, load_golden, same
 from vecgeom_amd import geometry as G
 from vecgeom_amd import workloads as W
 
@@ -45,19 +45,11 @@ EXTRA = [  # phi-sectioned and degenerate variants beyond the zoo
 ]
 
 
-# Distances of the Torus come from a quartic solved with cbrt/atan2/sin/cos;
-# library builds of those differ in the last ulp (host compilers differ from
-# each other already), so they are held to the FP64 distance tolerance
-# max(1e-9 mm, 1e-12 relative) instead of bit equality.
+# Distances of the Torus: the device solves the reference's quartic for its
+# exact real roots, the reference (and the oracle) in closed form; they are
+# held to the FP64 distance bar max(1e-9 mm, 1e-12 relative), and lanes
+# beyond it to the exact first crossing (conftest.assert_agree).
 TOLERANT = {("Torus", "dist_in"), ("Torus", "dist_out")}
-
-
-def close(a, b):
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
-    both = (a == b) | (np.isnan(a) & np.isnan(b))
-    with np.errstate(invalid="ignore"):
-        return both | (np.abs(a - b) <= np.maximum(1e-9, 1e-12 * np.abs(b)))
 
 
 class HostLib:
@@ -100,7 +92,10 @@ def compare(oracle, host, u, xf, inputs, family=""):
             args = (inputs["p"], inputs["d"] if name == "dist_in" else None, inputs["sm"] if name == "dist_in" else None)
         want = o.query(q, *[a for a in args if a is not None]) if args[1] is not None else o.query(q, args[0])
         got = run_host(host, nav, ids[0], hq, *args)
-        ok = close(got, want) if (family, name) in TOLERANT else same(got, want)
+        if (family, name) in TOLERANT:
+            assert_agree("Torus", name, got, want, "host", torus=(u.params, inputs["lp"], inputs["ld"]))
+            continue
+        ok = same(got, want)
         assert ok.all(), f"{name}: {int((~ok).sum())} lanes differ, first {np.flatnonzero(~ok)[:4]}"
 
 
