@@ -70,6 +70,15 @@ int vg_shutdown(void);
  *   VG_POLYCONE       sphi dphi nz rmin[nz] rmax[nz] z[nz] (UnplacedPolycone.h:82)
  *   VG_POLYHEDRON     phiStart phiDelta [deg] sides nz z[nz] rmin[nz] rmax[nz]
  *                                                      (UnplacedPolyhedron.h:169)
+ *   VG_BOOLEAN        op, then for the left and the right operand: type
+ *                     nparams params[nparams] placement[12]
+ *                     (UnplacedBooleanVolume(op, left, right),
+ *                     UnplacedBooleanVolume.h:39-50; op 0 union,
+ *                     1 intersection, 2 subtraction; each operand a
+ *                     primitive shape placed in the Boolean's frame,
+ *                     placement = translation then the 9 rotation entries).
+ *                     Basket and scalar shape queries only: batches and
+ *                     navigation refuse Boolean volumes.
  */
 enum vg_shape_type {
   VG_BOX = 1,
@@ -83,7 +92,8 @@ enum vg_shape_type {
   VG_PARABOLOID = 9,
   VG_TORUS = 10,
   VG_POLYCONE = 11,
-  VG_POLYHEDRON = 12
+  VG_POLYHEDRON = 12,
+  VG_BOOLEAN = 13
 };
 
 /* Returns 1 if the library has device kernels for this shape type. */

@@ -31,6 +31,7 @@
 #include "volumes/Torus.h"
 #include "volumes/Polycone.h"
 #include "volumes/Polyhedron.h"
+#include "volumes/UnplacedBooleanVolume.h"
 
 using namespace vecgeom;
 
@@ -94,6 +95,28 @@ VUnplacedVolume *MakeUnplaced(int type, const double *p, int n) {
       if (n != 4 + 3 * nz) break;
       std::vector<double> z(p + 4, p + 4 + nz), rmin(p + 4 + nz, p + 4 + 2 * nz), rmax(p + 4 + 2 * nz, p + 4 + 3 * nz);
       return new UnplacedPolyhedron(p[0], p[1], (int)p[2], nz, z.data(), rmin.data(), rmax.data());
+    }
+    case 13: {
+      // op (BooleanOperation), then per operand: type, nparams, params,
+      // placement[12]; operands placed with the reference's own classes
+      // (volumes/UnplacedBooleanVolume.h:39-50).
+      if (n < 1) break;
+      VPlacedVolume const *operand[2] = {nullptr, nullptr};
+      int at = 1;
+      for (int k = 0; k < 2; ++k) {
+        if (at + 2 > n) return nullptr;
+        const int t = (int)p[at], np = (int)p[at + 1];
+        if (t == 13 || np < 0 || at + 2 + np + 12 > n) return nullptr;
+        VUnplacedVolume *u = MakeUnplaced(t, p + at + 2, np);
+        if (!u) return nullptr;
+        const double *x = p + at + 2 + np;
+        auto *lv = new LogicalVolume(k == 0 ? "left" : "right", u);
+        operand[k] = lv->Place(new Transformation3D(x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], x[8], x[9], x[10],
+                                                    x[11]));
+        at += 2 + np + 12;
+      }
+      if (at != n || p[0] < 0 || p[0] > 2) break;
+      return new UnplacedBooleanVolume((BooleanOperation)(int)p[0], operand[0], operand[1]);
     }
   }
   return nullptr;

@@ -242,6 +242,7 @@ enum ShapeType {
   kTorus = 10,
   kPolycone = 11,
   kPolyhedron = 12,
+  kBoolean = 13,  // union / intersection / subtraction of two placed primitives (vgo_boolean.cpp)
 };
 
 // Factory: returns nullptr for an unknown type or a bad parameter count.

@@ -12,11 +12,13 @@ namespace vgo {
 Shape *MakeBox(const double *p, int n);
 Shape *MakeTube(const double *p, int n);
 Shape *MakeZooShape(int type, const double *p, int n);
+Shape *MakeBoolean(const double *p, int n);
 
 Shape *MakeShape(int type, const double *p, int n) {
   switch (type) {
     case kBox: return MakeBox(p, n);
     case kTube: return MakeTube(p, n);
+    case kBoolean: return MakeBoolean(p, n);
     default: return MakeZooShape(type, p, n);
   }
 }

@@ -134,7 +134,7 @@ def main():
         with open(mpath) as f:
             manifest = json.load(f)
     rng = np.random.Generator(np.random.Philox(W.SEED_BASE + 100))
-    shapes = W.config1_shapes() + W.zoo_shapes()
+    shapes = W.config1_shapes() + W.zoo_shapes() + W.boolean_shapes()
     seen = set()
     for idx, (name, u) in enumerate(shapes):
         if name in SKIP:

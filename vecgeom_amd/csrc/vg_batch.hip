@@ -847,6 +847,9 @@ int MakeTask(const vg_geometry *g, const vg_query_task &q, int i, TaskDev &t) {
   const PlacedRec &pr = g->placed[q.placed];
   if (!ShapeHasKernels(pr.type))
     return SetError(VG_ERR_UNSUPPORTED, "task " + std::to_string(i) + ": no device kernels for the shape");
+  if (pr.type == VG_BOOLEAN)
+    return SetError(VG_ERR_UNSUPPORTED, "task " + std::to_string(i) +
+                                            ": Boolean volumes run through the basket entry points, not batches");
   const int qc = QueryCode(q.query);
   if (qc < 0 || (q.input_type != VG_F64 && q.input_type != VG_F32 && q.input_type != VG_MIXED))
     return SetError(VG_ERR_INVALID, "task " + std::to_string(i) + ": bad query or input type");

@@ -10,6 +10,7 @@
 // its type tag, as the navigator does.
 #include <hip/hip_runtime.h>
 
+#include "shapes_boolean.hpp"
 #include "vg_nav_impl.hpp"
 
 namespace vga {
@@ -37,7 +38,7 @@ __global__ __launch_bounds__(kExtraBlock) void ContainsExtraKernel(const ExtraAr
   if (i >= a.n) return;
   const V3 p{a.x[i], a.y[i], a.z[i]};
   const V3 l = a.local_in ? p : XfPointRt(a.xf, a.xfkind, p);
-  a.out[i] = DContains(a.type, a.blk, l) ? 1 : 0;
+  a.out[i] = (a.type == VG_BOOLEAN ? BooleanS::Contains(BooleanP{a.blk}, l) : DContains(a.type, a.blk, l)) ? 1 : 0;
   if (a.lx) a.lx[i] = l.x;
   if (a.ly) a.ly[i] = l.y;
   if (a.lz) a.lz[i] = l.z;

@@ -35,6 +35,7 @@ bool ShapeHasKernels(int type) {
   switch (type) {
     case VG_BOX: case VG_TUBE: case VG_ORB: case VG_TRD: case VG_PARALLELEPIPED: case VG_PARABOLOID:
     case VG_TRAPEZOID: case VG_SPHERE: case VG_CONE: case VG_POLYCONE: case VG_TORUS: case VG_POLYHEDRON:
+    case VG_BOOLEAN:
       return true;
   }
   return false;
@@ -44,7 +45,7 @@ namespace {
 
 // Placement kind as Transformation3D::SetProperties decides it
 // (source/Transformation3D.cpp:81-90, 139-150).
-int PlacementKind(const double *xf12) {
+int KindOfPlacement(const double *xf12) {
   const bool hasT = std::fabs(xf12[0]) > kTolerance || std::fabs(xf12[1]) > kTolerance || std::fabs(xf12[2]) > kTolerance;
   int code = 0;
   for (int i = 0; i < 9; ++i) code |= (1 << i) * (std::fabs(xf12[3 + i]) > kTolerance);
@@ -163,7 +164,7 @@ int BuildGeometry(const vg_flat_geometry *f, vg_geometry *g) {
     r.logical = l;
     r.shape = g->logical_shape[l];
     r.type = g->shapes[r.shape].type;
-    r.xfkind = PlacementKind(x);
+    r.xfkind = KindOfPlacement(x);
     r.param_off = g->shapes[r.shape].param_off;
     r.dbegin = g->logical_dbegin[l];
     r.dend = g->logical_dbegin[l + 1];
@@ -180,7 +181,8 @@ int BuildGeometry(const vg_flat_geometry *f, vg_geometry *g) {
       g->extent32[6 * (size_t)p + i] = (float)lo[i];
       g->extent32[6 * (size_t)p + 3 + i] = (float)hi[i];
     }
-    if (!g->nav_bad_type && !ShapeHasKernels(r.type)) g->nav_bad_type = r.type;
+    // Navigation and serving dispatch primitive shapes only.
+    if (!g->nav_bad_type && (!ShapeHasKernels(r.type) || r.type == VG_BOOLEAN)) g->nav_bad_type = r.type;
     if (r.type >= 0 && r.type < 32) g->type_mask |= 1u << r.type;
   }
   // The daughters' float boxes in daughter-slot order: the navigator's box
@@ -194,5 +196,7 @@ int BuildGeometry(const vg_flat_geometry *f, vg_geometry *g) {
   }
   return VG_OK;
 }
+
+int PlacementKind(const double *xf12) { return KindOfPlacement(xf12); }
 
 }  // namespace vga

@@ -124,6 +124,11 @@ int LaunchContainsExtra(const vg_geometry *g, int32_t placed, const double *x, c
 
 // Launches one basket query on placed volume `placed` (vg_shape_kernels.hip).
 int LaunchQuery(const vg_geometry *g, int32_t placed, int q, const QArgs &a, hipStream_t s);
+// Basket queries on a Boolean volume (vg_boolean.hip).
+int LaunchBooleanQuery(const vg_geometry *g, int32_t placed, int q, const QArgs &a, hipStream_t s);
+// Placement kind (identity / translation / rotation / general) of 12
+// placement doubles, as Transformation3D::SetProperties decides it.
+int PlacementKind(const double *xf12);
 bool ShapeHasKernels(int type);
 int SetKernelVariant(int v);
 

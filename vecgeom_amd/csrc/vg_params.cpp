@@ -3,6 +3,7 @@
 // construction, computed with the same expressions on the host (glibc
 // cos/sin, IEEE double) so that device kernels see identical constants.
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 
 #include "shapes.hpp"
@@ -107,6 +108,38 @@ WedgeP MakeWedge(double angle, double zero) {
   return w;
 }
 
+// Boolean block (shapes_boolean.hpp): op, per operand type / block offset /
+// placement kind, total length, the two placements, then the operands'
+// blocks, each starting at an even offset (16-byte aligned).
+std::vector<double> BuildBooleanParams(const double *c, int n, std::string *err) {
+  std::vector<double> none;
+  if (n < 1 || !(c[0] == 0 || c[0] == 1 || c[0] == 2))
+    return Fail(err, "Boolean needs op 0 (union), 1 (intersection) or 2 (subtraction)"), none;
+  std::vector<double> blk(32, 0.);
+  blk[0] = c[0];
+  int at = 1;
+  for (int k = 0; k < 2; ++k) {
+    if (at + 2 > n) return Fail(err, "Boolean operand record truncated"), none;
+    const int t = (int)c[at], np = (int)c[at + 1];
+    if (t == VG_BOOLEAN) return Fail(err, "Boolean operands must be primitive shapes"), none;
+    if (np < 0 || at + 2 + np + 12 > n) return Fail(err, "Boolean operand record truncated"), none;
+    std::vector<double> ob = BuildParams(t, c + at + 2, np, err);
+    if (ob.empty()) return none;
+    const double *x = c + at + 2 + np;
+    blk.resize((blk.size() + 1) & ~size_t(1), 0.);
+    blk[1 + 3 * k] = t;
+    blk[2 + 3 * k] = (double)blk.size();
+    blk[3 + 3 * k] = PlacementKind(x);
+    for (int i = 0; i < 12; ++i) blk[8 + 12 * k + i] = x[i];
+    blk.insert(blk.end(), ob.begin(), ob.end());
+    at += 2 + np + 12;
+  }
+  if (at != n) return Fail(err, "Boolean parameter count does not match its operands"), none;
+  blk.resize((blk.size() + 1) & ~size_t(1), 0.);
+  blk[7] = (double)blk.size();
+  return blk;
+}
+
 std::vector<double> BuildParams(int type, const double *c, int n, std::string *err) {
   std::vector<double> none;
   switch (type) {
@@ -122,6 +155,8 @@ std::vector<double> BuildParams(int type, const double *c, int n, std::string *e
         return Fail(err, "tube needs 0 <= rmin < rmax, dz > 0, dphi > 0"), none;
       return Pack(MakeTube(c[0], c[1], c[2], c[3], c[4]));
     }
+    case VG_BOOLEAN:
+      return BuildBooleanParams(c, n, err);
     default:
       return BuildZooParams(type, c, n, err);
   }
@@ -140,6 +175,25 @@ bool ShapeExtent(int type, const double *b, double lo[3], double hi[3]) {
     case VG_TUBE: {
       const TubeP *p = reinterpret_cast<const TubeP *>(b);
       TubeExtent(*p, lo, hi);
+      return true;
+    }
+    case VG_BOOLEAN: {
+      // source/UnplacedBooleanVolume.cpp:94-122: the operands' unplaced
+      // extents (their placements not applied), combined per operation.
+      double la[3], ha[3], lb[3], hb[3];
+      if (!ShapeExtent((int)b[1], b + (int)b[2], la, ha) || !ShapeExtent((int)b[4], b + (int)b[5], lb, hb)) return false;
+      for (int i = 0; i < 3; ++i) {
+        if (b[0] == 0) {
+          lo[i] = std::min(la[i], lb[i]);
+          hi[i] = std::max(ha[i], hb[i]);
+        } else if (b[0] == 1) {
+          lo[i] = std::max(la[i], lb[i]);
+          hi[i] = std::min(ha[i], hb[i]);
+        } else {
+          lo[i] = la[i];
+          hi[i] = ha[i];
+        }
+      }
       return true;
     }
     default:

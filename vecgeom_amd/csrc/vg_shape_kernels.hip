@@ -491,6 +491,7 @@ int LaunchQuery(const vg_geometry *g, int32_t placed, int q, const QArgs &a, hip
   const PlacedRec &pr = g->placed[placed];
   const double *block = g->params.data() + pr.param_off;
   switch (pr.type) {
+    case VG_BOOLEAN: return LaunchBooleanQuery(g, placed, q, a, s);
     case VG_BOX: return LaunchShape<BoxS>(g, placed, block, pr, q, a, s);
     case VG_TUBE: return LaunchShape<TubeS>(g, placed, block, pr, q, a, s);
     case VG_ORB: return LaunchShape<OrbS>(g, placed, block, pr, q, a, s);

@@ -22,11 +22,13 @@ from . import capi
 
 VG_BOX, VG_TUBE, VG_ORB, VG_SPHERE, VG_CONE, VG_TRD, VG_TRAPEZOID = 1, 2, 3, 4, 5, 6, 7
 VG_PARALLELEPIPED, VG_PARABOLOID, VG_TORUS, VG_POLYCONE, VG_POLYHEDRON = 8, 9, 10, 11, 12
+VG_BOOLEAN = 13
+kUnion, kIntersection, kSubtraction = 0, 1, 2  # BooleanOperation, volumes/UnplacedBooleanVolume.h:10-14
 
 SHAPE_NAMES = {
     VG_BOX: "Box", VG_TUBE: "Tube", VG_ORB: "Orb", VG_SPHERE: "Sphere", VG_CONE: "Cone", VG_TRD: "Trd",
     VG_TRAPEZOID: "Trapezoid", VG_PARALLELEPIPED: "Parallelepiped", VG_PARABOLOID: "Paraboloid",
-    VG_TORUS: "Torus", VG_POLYCONE: "Polycone", VG_POLYHEDRON: "Polyhedron",
+    VG_TORUS: "Torus", VG_POLYCONE: "Polycone", VG_POLYHEDRON: "Polyhedron", VG_BOOLEAN: "Boolean",
 }
 
 kDegToRad = math.pi / 180.0
@@ -78,6 +80,22 @@ def UnplacedParallelepiped(x, y, z, alpha, theta, phi):
 
 def UnplacedParaboloid(rlo, rhi, dz):
     return UnplacedVolume(VG_PARABOLOID, [rlo, rhi, dz])
+
+
+def UnplacedBooleanVolume(op, left, left_xf, right, right_xf):
+    """UnplacedBooleanVolume(op, leftPlaced, rightPlaced)
+    (volumes/UnplacedBooleanVolume.h:39-50): the operands are primitive
+    shapes placed in the Boolean's own frame, given here as (unplaced,
+    Transformation3D) pairs.  Parameter block: op, then per operand type,
+    number of parameters, parameters, placement (12 doubles)."""
+    if op not in (kUnion, kIntersection, kSubtraction):
+        raise ValueError("op must be kUnion, kIntersection or kSubtraction")
+    params = [op]
+    for u, xf in ((left, left_xf), (right, right_xf)):
+        if u.type == VG_BOOLEAN:
+            raise ValueError("Boolean operands must be primitive shapes")
+        params += [u.type, len(u.params)] + list(u.params) + list(xf.array())
+    return UnplacedVolume(VG_BOOLEAN, params)
 
 
 def UnplacedTorus(rmin, rmax, rtor, sphi, dphi):
@@ -284,20 +302,6 @@ class DeviceGeometry:
             self.free()
         except Exception:
             pass
-
-
-def single_shape_flat(unplaced_list, transforms, world_half=1e4):
-    """Flat arrays of a world box holding one placement per (shape,
-    transform); returns (FlatArrays, [placed ids], GeoManager)."""
-    gm = GeoManager()
-    world_lv = LogicalVolume("world", UnplacedBox(world_half, world_half, world_half), gm)
-    ids = []
-    for k, (u, xf) in enumerate(zip(unplaced_list, transforms)):
-        lv = LogicalVolume(f"shape{k}", u, gm)
-        pv = world_lv.PlaceDaughter(f"shape{k}", lv, xf)
-        ids.append(pv.id)
-    gm.SetWorld(world_lv.Place("world"))
-    return gm.CloseGeometry(), ids, gm
 
 
 def single_shape_flat(unplaced_list, transforms, world_half=1e4):

@@ -49,6 +49,32 @@ def zoo_shapes():
     ]
 
 
+def boolean_shapes():
+    """Boolean volumes (UnplacedBooleanVolume): union, intersection and
+    subtraction with partially overlapping operands, a subtraction that cuts
+    a hole, and an intersection with a Polyhedron operand (an operand with a
+    variable-length parameter block).  The left operand sits at the
+    Boolean's origin, as in the reference's examples; the right one is
+    translated and rotated."""
+    shapes = dict(zoo_shapes())
+    ident = G.Transformation3D()
+    rot = G.Transformation3D.from_array(
+        [10.0, -5.0, 4.0] + list(random_rotation(np.random.Generator(np.random.Philox(SEED_BASE + 40)))))
+    box = G.UnplacedBox(30.0, 20.0, 25.0)
+    return [
+        ("BoolUnion", G.UnplacedBooleanVolume(G.kUnion, box, ident, G.UnplacedTube(5.0, 20.0, 30.0, 0.0, TWO_PI),
+                                              G.Transformation3D(15.0, 5.0, -3.0))),
+        ("BoolInter", G.UnplacedBooleanVolume(G.kIntersection, box, ident, G.UnplacedOrb(28.0), rot)),
+        ("BoolSub", G.UnplacedBooleanVolume(G.kSubtraction, box, ident, G.UnplacedTube(5.0, 20.0, 30.0, 0.0, TWO_PI),
+                                            rot)),
+        ("BoolHole", G.UnplacedBooleanVolume(G.kSubtraction, G.UnplacedCone(0.0, 30.0, 0.0, 20.0, 30.0, 0.0, TWO_PI),
+                                             ident, G.UnplacedTube(0.0, 6.0, 8.0, 0.0, TWO_PI),
+                                             G.Transformation3D(3.0, 2.0, -4.0))),
+        ("BoolPolyInter", G.UnplacedBooleanVolume(G.kIntersection, shapes["Polyhedron"], ident,
+                                                  G.UnplacedBox(25.0, 25.0, 25.0), G.Transformation3D(5.0, 5.0, 5.0))),
+    ]
+
+
 def random_rotation(rng):
     """Uniform random rotation (Shoemake's quaternion method); returns
     r0..r8 with master = t + R local."""
