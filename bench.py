@@ -115,6 +115,7 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched and FP32-storage forms")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision workload and forms")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage batched form")
+    ap.add_argument("--no-boolean", action="store_true", help="skip the Boolean-volume queries")
     ap.add_argument("--nav-tracks", type=int, default=1 << 20, help="tracks of the nested-geometry navigation run")
     ap.add_argument("--nav-steps", type=int, default=3, help="timed trace launches")
     ap.add_argument("--no-nav", action="store_true")
@@ -335,6 +336,71 @@ def time_steps(launches, steps, stream, watch=None):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     return elapsed, {k: [a.elapsed_time(b) for a, b in v] for k, v in ev.items()}
+
+
+def boolean_bench(args, dev, stream, with_cpu):
+    """Boolean volumes (union, intersection, subtraction; see
+    workloads.boolean_shapes): each query on 2^20 points uniform in a 120 mm
+    cube around the volume, random placement, per-launch kernels timed with
+    HIP events (mean of the timed launches); Contains / Inside / SafetyToIn /
+    DistanceToIn on every point, DistanceToOut / SafetyToOut on the points
+    inside.  With the CPU leg: the reference's scalar virtuals on one host
+    thread over a strided sample of the same queries."""
+    import torch
+
+    from vecgeom_amd import geometry as G
+    from vecgeom_amd import queries as Q
+    from vecgeom_amd import workloads as W
+
+    n = 1 << 20
+    shapes = W.boolean_shapes()
+    rng = np.random.Generator(np.random.Philox(W.SEED_BASE + 77))
+    xfs = [W.random_placement(rng) for _ in shapes]
+    geom, ids, _ = G.single_shape_geometry([u for _, u in shapes], xfs)
+    t = lambda a: tuple(torch.from_numpy(np.ascontiguousarray(a[i])).to(dev) for i in range(3))
+    rows = []
+    ref = None
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        ref = po.reference(fast=True)
+    for (name, u), xf, pid in zip(shapes, xfs, ids):
+        inp = W.shape_inputs(((-60, -60, -60), (60, 60, 60)), xf, n, seed=W.SEED_BASE + 700 + pid)
+        p, d = t(inp["p"]), t(inp["d"])
+        c = Q.contains(geom, pid, p)
+        torch.cuda.synchronize()
+        inside = np.flatnonzero(c.cpu().numpy() == 1)
+        lp, ld = t(inp["lp"][:, inside]), t(inp["ld"][:, inside])
+        forms = [("contains", lambda: Q.contains(geom, pid, p, out=c), n),
+                 ("inside", lambda: Q.inside(geom, pid, p), n),
+                 ("dist_in", lambda: Q.distance_to_in(geom, pid, p, d), n),
+                 ("safe_in", lambda: Q.safety_to_in(geom, pid, p), n),
+                 ("dist_out", lambda: Q.distance_to_out(geom, pid, lp, ld), len(inside)),
+                 ("safe_out", lambda: Q.safety_to_out(geom, pid, lp), len(inside))]
+        for qn, fn, m in forms:
+            if m == 0:
+                continue
+            for _ in range(2):
+                fn()
+            _, ms = time_steps([((name, qn), fn)], 5, stream)
+            us = 1e3 * float(np.mean(ms[0]))
+            rows.append({"shape": name, "query": qn, "n": m, "avg_us": round(us, 3),
+                         "gqps": round(m / us / 1e3, 3)})
+        if ref is not None:
+            h = ref.placed(u.type, u.params, xf.array())
+            sel = np.arange(0, n, 64)
+            t0 = time.perf_counter()
+            h.query(po.Q_CONTAINS, inp["p"][:, sel])
+            h.query(po.Q_DIST_IN, inp["p"][:, sel], inp["d"][:, sel], np.full(len(sel), 1e30))
+            h.query(po.Q_SAFE_IN, inp["p"][:, sel])
+            dt = time.perf_counter() - t0
+            rows[-1]["cpu_ref_1thread_gqps_contains_distin_safein"] = round(3 * len(sel) / dt / 1e9, 5)
+    geom.free()
+    tot_q = sum(r["n"] for r in rows)
+    tot_us = sum(r["avg_us"] for r in rows)
+    return {"workload": "Boolean volumes (union, intersection, subtraction, hole, Polyhedron intersection), "
+                        "random placements, 2^20 points per volume, six queries, one launch per (volume, query)",
+            "value": round(tot_q / tot_us / 1e3, 3), "unit": "Gqueries/s", "per_kernel": rows}
 
 
 def host_cpu():
@@ -1048,6 +1114,11 @@ def main():
         log("navigation")
         nav = nav_bench(args, dev, stream, not args.no_cpu)
 
+    boolean = None
+    if not args.no_boolean:
+        log("Boolean volumes")
+        boolean = boolean_bench(args, dev, stream, not args.no_cpu)
+
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gqueries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
@@ -1067,6 +1138,7 @@ def main():
         "cpu_baseline": cpu,
         "nav": nav,
         "serve": serve,
+        "boolean": boolean,
         "per_kernel": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()} for r in rows],
     }
     print(json.dumps(line), flush=True)
