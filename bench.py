@@ -42,12 +42,17 @@ PROFILE = os.path.join(ROOT, "profiles", "shape_zoo_fp64_kernels.md")
 BATCH_PROFILE = os.path.join(ROOT, "profiles", "shape_zoo_batched_kernels.md")
 
 
-# Weight classes of the batched launch (vg_batch.hip ClassOf; the
-# variable-length shapes' blocks of this workload fit in LDS: class 3).
+# Weight classes of the batched launch (vg_batch.hip ClassOf / TaskClass; the
+# variable-length shapes' blocks of this workload fit in LDS: class 3; the
+# Torus's Contains, Inside and safeties run in the medium class).
 SHAPE_CLASS = {"Cone": 1, "Sphere": 1, "Torus": 2, "Polycone": 3, "Polyhedron": 3}
 
 
-def shape_class(name):
+def shape_class(name, query=None):
+    """Weight class of a (shape, batch query code) task."""
+    from vecgeom_amd import batch as B
+    if name == "Torus" and query is not None and query not in B.DIRECTIONAL:
+        return 1
     return SHAPE_CLASS.get(name, 0)
 
 
@@ -608,7 +613,7 @@ def batch_forms(args, wl, stream):
     # kernel of the step.
     alone = []
     for c in classes:
-        sub = [t for t in wl.tasks() if shape_class(wl.names_of_pid[t.placed]) == c["class"]]
+        sub = [t for t in wl.tasks() if shape_class(wl.names_of_pid[t.placed], t.query) == c["class"]]
         if not sub:
             continue
         cb = B.Batch(wl.geom, sub)

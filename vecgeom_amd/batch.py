@@ -59,7 +59,8 @@ capi.register({
 
 # Weight classes of a batch (one kernel launch each); mixed-precision tasks
 # run in classes of their own, 5 + the base class.
-_BASE_CLASS_NAMES = ["light (Box, Tube, Orb, Trd, Trapezoid, Parallelepiped, Paraboloid)", "Cone, Sphere", "Torus",
+_BASE_CLASS_NAMES = ["light (Box, Tube, Orb, Trd, Trapezoid, Parallelepiped, Paraboloid)",
+                     "Cone, Sphere; Torus Contains, Inside and safeties", "Torus distances",
                      "Polycone, Polyhedron (parameters staged in LDS)",
                      "Polycone, Polyhedron (parameters in global memory)"]
 CLASS_NAMES = _BASE_CLASS_NAMES + [f"mixed: {n}" for n in _BASE_CLASS_NAMES]

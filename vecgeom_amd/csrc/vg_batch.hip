@@ -320,6 +320,11 @@ constexpr bool MixedClass(int c) { return c >= kBaseClasses; }
 // zoo step (7 waves: 99, 8: 93).  The Torus class measured best unbounded.
 constexpr int ClassWaves(int c) { return Base(c) == kStagedClass ? 6 : 1; }
 
+// Queries whose code is short for every shape: a Torus task of these runs in
+// the medium class, away from the register budget (and low occupancy) of the
+// Torus class's quartic.
+constexpr bool CheapQuery(int q) { return q == kQContains || q == kQInside || q == kQSafeIn || q == kQSafeOut; }
+
 constexpr int ClassOf(int type) {
   return (type == VG_CONE || type == VG_SPHERE)           ? 1
          : (type == VG_TORUS)                              ? 2
@@ -343,6 +348,17 @@ constexpr uint64_t TileItems(int c, int it, int type, int query) {
   return (uint64_t)kBatchBlock * ClassIT(c, it) * ((Base(c) == kStagedClass && WalkTask(type, query)) ? kWalkTileMul : 1);
 }
 
+// The four short queries of a shape (CheapQuery), FP64 or FP32 storage.
+template <class S, int IT>
+__device__ __forceinline__ void RunShapeCheap(const TaskDev t, const double *blk, uint64_t i0) {
+  switch (t.query) {
+    case kQContains: RunTile<S, kQContains, IT, false>(t, blk, i0); break;
+    case kQInside: RunTile<S, kQInside, IT, false>(t, blk, i0); break;
+    case kQSafeIn: RunTile<S, kQSafeIn, IT, false>(t, blk, i0); break;
+    case kQSafeOut: RunTile<S, kQSafeOut, IT, false>(t, blk, i0); break;
+  }
+}
+
 template <int IT, int CLS>
 __device__ __forceinline__ void RunTask(const TaskDev t, const double *blk, uint64_t i0) {
   constexpr int B = Base(CLS);
@@ -361,6 +377,9 @@ __device__ __forceinline__ void RunTask(const TaskDev t, const double *blk, uint
     switch (t.type) {
       case VG_SPHERE: RunShape<SphereS, IT, MX>(t, blk, i0); break;
       case VG_CONE: RunShape<ConeS, IT, MX>(t, blk, i0); break;
+      case VG_TORUS:
+        if constexpr (!MX) RunShapeCheap<TorusS, IT>(t, blk, i0);
+        break;
     }
   } else if constexpr (B == 2) {
     if (t.type == VG_TORUS) RunShape<TorusS, IT, MX>(t, blk, i0);
@@ -677,6 +696,7 @@ bool Filtered(const TaskDev &t) {
 
 int TaskClass(const TaskDev &t) {
   int c = ClassOf(t.type);
+  if (t.type == VG_TORUS && CheapQuery(t.query) && !Filtered(t)) c = 1;
   if (c == kStagedClass && t.param_len > kMaxStageDoubles) c = kStagedClass + 1;
   return c + (Filtered(t) ? kBaseClasses : 0);
 }
