@@ -226,7 +226,32 @@ struct Shape {
   // Shapes whose placed DistanceToIn does not use TransformDirection (the
   // reference's Torus) override this; default is the correct form.
   virtual bool DirectionUsesPointTransform() const { return false; }
+  // VPlacedVolume::Normal(localPoint, normal&) (volumes/PlacedVolume.h:266-272
+  // and the shapes' overrides).  n starts as a zero vector (a default
+  // Vector3D) and is set as the reference sets it; returns the reference's
+  // valid flag (1 or 0), or -1 where the reference has no Normal for the
+  // shape (the base class asserts).
+  virtual int Normal(V3 p, V3 &n) const {
+    (void)p;
+    n = V3{0, 0, 0};
+    return -1;
+  }
 };
+
+// The USolids DistanceToOut(point, direction, normal&, convex&, stepMax)
+// (volumes/ShapeImplementationHelper.h:277-295, identical in
+// ScalarShapeImplementationHelper.h:249-264): DistanceToOut, then the Normal
+// of point + d * direction, convex always true.  Returns the distance; valid
+// as Normal returns it.
+inline double DistanceToOutNormal(const Shape &s, V3 p, V3 d, double stepMax, V3 &n, bool &convex, int &valid) {
+  const double dist = s.DistanceToOut(p, d, stepMax);
+  // Vector3D operator*(scalar, vector) scales a copy of the vector; then
+  // point + that, component by component.
+  const V3 hit{p.x + d.x * dist, p.y + d.y * dist, p.z + d.z * dist};
+  valid = s.Normal(hit, n);
+  convex = true;
+  return dist;
+}
 
 // Shape type tags shared with the product C-ABI (include/vecgeom_amd.h).
 enum ShapeType {

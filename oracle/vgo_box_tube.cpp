@@ -11,6 +11,27 @@ struct Box : Shape {
   double d[3];
   Box(double dx, double dy, double dz) : d{dx, dy, dz} {}
 
+  // BoxImplementation::NormalKernel, BoxImplementation.h:842-898: every face
+  // within 100 kTolerance of the point adds the sign of that coordinate to
+  // its axis; two or three faces (an edge, a corner) are scaled by 1/sqrt(2)
+  // or 1/sqrt(3); valid when at least one face is that close.
+  int Normal(V3 p, V3 &n) const override {
+    const double delta = 100. * kTolerance;
+    const double kInvSqrt2 = 0.7071067811865475, kInvSqrt3 = 0.5773502691896258;
+    n = V3{0, 0, 0};
+    double nsurf = 0;
+    for (int dim = 0; dim < 3; ++dim) {
+      const double safe = Abs(Abs(p[dim]) - d[dim]);
+      if (safe < delta) {
+        nsurf = nsurf + 1.;
+        n[dim] = n[dim] + (p[dim] < 0 ? -1. : 1.);
+      }
+    }
+    if (nsurf == 3.) n = n * kInvSqrt3;
+    if (nsurf == 2.) n = n * kInvSqrt2;
+    return nsurf > 0 ? 1 : 0;
+  }
+
   // GenericKernelForContainsAndInside, BoxImplementation.h:608-653
   template <bool ForInside>
   void Kernel(V3 p, bool &cin, bool &cout) const {
@@ -218,6 +239,70 @@ struct Tube : Shape {
       if (ForInside) cin &= cinphi;
     }
   }
+
+  // UnplacedTube::Normal, source/UnplacedTube.cpp:141-229.  Surfaces are
+  // tested in the order top, bottom, inner wall, outer wall, the two phi
+  // planes; the first one found sets the vector, later ones add to it, and
+  // the sum is divided by sqrt(count) when there are several.  The z test
+  // takes kTolerance; the radial range of the lids is the exact one.
+  int Normal(V3 p, V3 &n) const override {
+    n = V3{0, 0, 0};
+    int ns = 0;
+    const double x2y2 = Sqrt(p.x * p.x + p.y * p.y);
+    const bool inZ = (p.z < z + kTolerance) && (p.z > -z - kTolerance);
+    const bool inR = (x2y2 >= rmin) && (x2y2 <= rmax);
+    if (inR && Abs(p.z - z) <= kTolerance) {
+      n = V3{0., 0., 1.};
+      ns++;
+    }
+    if (inR && Abs(p.z + z) <= kTolerance) {
+      if (ns > 0)
+        n.z += -1;
+      else
+        n = V3{0.0, 0.0, -1};
+      ns++;
+    }
+    if (rmin > 0.) {
+      if (inZ && Abs(x2y2 - rmin) <= kTolerance) {
+        const double inv = 1. / x2y2;
+        if (ns == 0) {
+          n = V3{-p.x * inv, -p.y * inv, 0.0};
+        } else {
+          n.x += -p.x * inv;
+          n.y += -p.y * inv;
+        }
+        ns++;
+      }
+    }
+    if (inZ && Abs(x2y2 - rmax) <= kTolerance) {
+      const double inv = 1. / x2y2;
+      if (ns > 0) {
+        n.x += p.x * inv;
+        n.y += p.y * inv;
+      } else {
+        n = V3{p.x * inv, p.y * inv, 0.0};
+      }
+      ns++;
+    }
+    if (dphi < kTwoPi) {
+      // -fPhiWedge.GetNormal1/2(): the wedge's inward normals negated (z -0).
+      const V3 o1{-wedge.n1x, -wedge.n1y, -0.0}, o2{-wedge.n2x, -wedge.n2y, -0.0};
+      if (inR && Wedge::OnSurface(wedge.a1x, wedge.a1y, wedge.n1x, wedge.n1y, p)) {
+        n = (ns == 0) ? o1 : n + o1;
+        ns++;
+      }
+      if (inR && Wedge::OnSurface(wedge.a2x, wedge.a2y, wedge.n2x, wedge.n2y, p)) {
+        n = (ns == 0) ? o2 : n + o2;
+        ns++;
+      }
+    }
+    if (ns > 1) {
+      const double s = std::sqrt((double)ns);
+      n = V3{n.x / s, n.y / s, n.z / s};
+    }
+    return ns != 0 ? 1 : 0;
+  }
+
   bool Contains(V3 p) const override {
     bool cin = false, cout;
     Kernel<false>(p, cin, cout);

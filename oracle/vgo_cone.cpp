@@ -73,6 +73,90 @@ struct Cone : Shape {
     rminTreatment = rmin1 > 0 || rmin2 > 0;
   }
 
+  // The two normals of the reference's cone: ConeImplementation::NormalKernel
+  // (ConeImplementation.h:2002-2128, PlacedCone::Normal) and
+  // UnplacedCone::Normal (source/UnplacedCone.cpp:31-141, what a Polycone
+  // section answers).  They differ only in how they scale by sec(alpha):
+  // the kernel multiplies by the cached inverse where the unplaced form
+  // divides.  Surfaces within 0.5 kTolerance (the phi planes: half the
+  // angular tolerance, the same number) are summed in the order outer cone,
+  // inner cone, start phi, end phi, z lid; unit length when several.
+  template <bool Kernel_>
+  int NormalT(V3 p, V3 &n) const {
+    int noSurfaces = 0;
+    double distSPhi = kInfinity, distEPhi = kInfinity;
+    const double delta = 0.5 * kTolerance, dAngle = 0.5 * kAngTol;
+    V3 sumnorm{0., 0., 0.}, nR{0, 0, 0}, nr{0., 0., 0.}, nPs{0, 0, 0}, nPe{0, 0, 0};
+    const V3 nZ{0., 0., 1.};
+    const double distZ = Abs(Abs(p.z) - dz);
+    const double rho = Sqrt(p.x * p.x + p.y * p.y);
+    const double pRMin = rho - p.z * tanRMin;
+    const double widRMin = rmin2 - dz * tanRMin;
+    const double distRMin = Kernel_ ? Abs(pRMin - widRMin) * invSecRMin : Abs(pRMin - widRMin) / secRMin;
+    const double pRMax = rho - p.z * tanRMax;
+    const double widRMax = rmax2 - dz * tanRMax;
+    const double distRMax = Kernel_ ? Abs(pRMax - widRMax) * invSecRMax : Abs(pRMax - widRMax) / secRMax;
+    if (!fullPhi) {
+      if (rho != 0.) {
+        // The kernel calls the backend's ATan2, the unplaced form std::atan2.
+        double pPhi = Kernel_ ? ATan2(p.y, p.x) : std::atan2(p.y, p.x);
+        if (pPhi < sphi - delta)
+          pPhi += kTwoPi;
+        else if (pPhi > sphi + dphi + delta)
+          pPhi -= kTwoPi;
+        distSPhi = Abs(pPhi - sphi);
+        distEPhi = Abs(pPhi - sphi - dphi);
+      } else if (!(rmin1 != 0.) || !(rmin2 != 0.)) {
+        distSPhi = 0.;
+        distEPhi = 0.;
+      }
+      nPs = V3{sinSPhi, -cosSPhi, 0};
+      nPe = V3{-sinEPhi, cosEPhi, 0};
+    }
+    if (rho > delta) {
+      nR = V3{p.x / rho / secRMax, p.y / rho / secRMax, Kernel_ ? -tanRMax * invSecRMax : -tanRMax / secRMax};
+      if (rmin1 != 0. || rmin2 != 0.)
+        nr = V3{-p.x / rho / secRMin, -p.y / rho / secRMin, Kernel_ ? tanRMin * invSecRMin : tanRMin / secRMin};
+    }
+    if (distRMax <= delta) {
+      noSurfaces++;
+      sumnorm = sumnorm + nR;
+    }
+    if ((rmin1 != 0. || rmin2 != 0.) && (distRMin <= delta)) {
+      noSurfaces++;
+      sumnorm = sumnorm + nr;
+    }
+    if (!fullPhi) {
+      if (distSPhi <= dAngle) {
+        noSurfaces++;
+        sumnorm = sumnorm + nPs;
+      }
+      if (distEPhi <= dAngle) {
+        noSurfaces++;
+        sumnorm = sumnorm + nPe;
+      }
+    }
+    if (distZ <= delta) {
+      noSurfaces++;
+      if (p.z >= 0.)
+        sumnorm = sumnorm + nZ;
+      else
+        sumnorm = sumnorm - nZ;
+    }
+    if (noSurfaces == 0) {
+      n = V3{0, 0, 0};  // the kernel's default vector, the unplaced form's zero sum
+    } else if (noSurfaces == 1) {
+      n = sumnorm;
+    } else {
+      // Vector3D::Unit: divide by Sqrt(Mag2 + kMinimum).
+      const double s = Sqrt(sumnorm.Mag2() + kMinimum);
+      n = V3{sumnorm.x / s, sumnorm.y / s, sumnorm.z / s};
+    }
+    return noSurfaces != 0 ? 1 : 0;
+  }
+  int Normal(V3 p, V3 &n) const override { return NormalT<true>(p, n); }
+  int UnplacedNormal(V3 p, V3 &n) const { return NormalT<false>(p, n); }
+
   // GenericKernelForContainsAndInside, early returns as the Scalar backend.
   template <bool ForInside>
   void Kernel(V3 p, bool &cin, bool &cout) const {
@@ -613,6 +697,21 @@ struct Polycone : Shape {
     return -2;
   }
   static V3 Shift(V3 p, double s) { return V3{p.x - 0., p.y - 0., p.z - s}; }
+
+  // UnplacedPolycone::Normal, source/UnplacedPolycone.cpp:684-698: -z or +z
+  // below or above the sections (valid), else the section cone's
+  // UnplacedCone::Normal of the point as given -- the reference does not
+  // shift it into the section's frame.
+  int Normal(V3 p, V3 &n) const override {
+    const int index = SectionIndex(p.z);
+    n = V3{0, 0, 0};
+    if (index < 0) {
+      if (index == -1) n = V3{0., 0., -1.};
+      if (index == -2) n = V3{0., 0., 1.};
+      return 1;
+    }
+    return sec[index].UnplacedNormal(p, n);
+  }
 
   bool Contains(V3 p) const override {
     if (p.z < zs[0] || p.z > zs[NSec()]) return false;

@@ -24,6 +24,14 @@ struct Orb : Shape {
     rTolI = r - rTol;
     rTolO = r + rTol;
   }
+  // OrbImplementation::NormalKernel, OrbImplementation.h:208-229: the radial
+  // direction, valid within the radial tolerance band.
+  int Normal(V3 p, V3 &n) const override {
+    const double rad2 = p.Mag2();
+    const double rad = Sqrt(rad2);
+    n = V3{p.x / rad, p.y / rad, p.z / rad};
+    return ((rad2 <= rTolO * rTolO) && (rad2 >= rTolI * rTolI)) ? 1 : 0;
+  }
   template <bool ForInside>
   void Kernel(V3 p, bool &cin, bool &cout) const {
     double rad2 = p.Mag2();
@@ -109,6 +117,41 @@ struct Trd : Shape {
     fy = 0.5 * (dy1 - dy2) / dz;
     calfx = 1. / Sqrt(1.0 + fx * fx);
     calfy = 1. / Sqrt(1.0 + fy * fy);
+  }
+  // UnplacedTrd::Normal, source/UnplacedTrd.cpp:154-199, as it executes.
+  // Each of its six "if (onSurf && FacX) norm[0] = ...; norm[1] = ...;
+  // norm[2] = ...; nosurface++;" lines guards only the first assignment, and
+  // each Fac is a double product converted to bool (true when nonzero or
+  // NaN).  So x takes the x-component of the last face whose Fac is true, y
+  // and z those of the -z face, and the result is always valid.
+  int Normal(V3 p, V3 &n) const override {
+    const double xp = p.x, yp = p.y, zp = p.z;
+    const double xx1 = dx1, xx2 = dx2, yy1 = dy1, yy2 = dy2, zz = dz;
+    const double XplusX = 2.0 * yy1 * 2.0 * zz, XminusX = XplusX;
+    const double XplusY = 0.0, XminusY = 0.0;
+    const double XplusZ = 2.0 * yy1 * (xx1 - xx2), XminusZ = 2.0 * yy1 * (-xx1 + xx2);
+    const double YplusX = 0.0, YminusX = 0.0;
+    const double YplusY = -2.0 * xx1 * 2.0 * zz, YminusY = YplusY;
+    const double YplusZ = 2.0 * xx1 * (-yy1 + yy2), YminusZ = 2.0 * xx1 * (yy1 - yy2);
+    const double ZplusX = 0.0, ZminusX = 0.0, ZplusY = 0.0, ZminusY = 0.0;
+    const double ZplusZ = -2.0 * xx2 * 2.0 * yy2, ZminusZ = 2.0 * xx2 * 2.0 * yy1;
+    const bool fpx = (XplusX * (xp - xx2) + XplusY * (yp - yy2) + XplusZ * (zp - zz)) != 0;
+    const bool fmx = (XminusX * (xp + xx2) + XminusY * (yp - yy2) + XminusZ * (zp - zz)) != 0;
+    const bool fpy = (YplusX * (xp - xx2) + YplusY * (yp - yy2) + YplusZ * (zp - zz)) != 0;
+    const bool fmy = (YplusX * (xp - xx2) + YminusY * (yp + yy2) + YminusZ * (zp - zz)) != 0;
+    const bool fpz = (ZplusX * (xp - xx2) + ZplusY * (yp - yy2) + ZplusZ * (zp - zz)) != 0;
+    const bool fmz = (ZminusX * (xp - xx2) + ZminusY * (yp + yy2) + ZminusZ * (zp - zz)) != 0;
+    const bool onSurf = fpx || fmx || fpy || fmy || fpz || fmz;
+    n = V3{0, 0, 0};
+    if (onSurf && fpx) n.x = XplusX;
+    if (onSurf && fmx) n.x = XminusX;
+    if (onSurf && fpy) n.x = YplusX;
+    if (onSurf && fmy) n.x = YminusX;
+    if (onSurf && fpz) n.x = ZplusX;
+    if (onSurf && fmz) n.x = ZminusX;
+    n.y = ZminusY;
+    n.z = ZminusZ;
+    return 1;
   }
   // TrdUtilities::PointLineOrientation :40-46
   static double Cross(double px, double py, double vx, double vy) { return vx * py - vy * px; }

@@ -65,6 +65,179 @@ struct Sphere : Shape {
     tanETheta = std::tan(eTheta);
     tanETheta2 = tanETheta * tanETheta;
   }
+
+  // The reference's accessors (volumes/UnplacedSphere.h:262-270) return
+  // swapped flags: IsFullSphere() is the phi flag, IsFullPhiSphere() the
+  // flag for both.  The normal kernels call them by these names.
+  bool IsFullSphere() const { return fullPhi; }
+  bool IsFullPhiSphere() const { return fullPhi && fullTheta; }
+  bool IsFullThetaSphere() const { return fullTheta; }
+
+  // SphereImplementation::ApproxSurfaceNormalKernel,
+  // SphereImplementation.h:463-610: the normal of the nearest surface for a
+  // point on none (r shells, then phi planes, then theta cones; each later
+  // candidate must be strictly nearer).
+  V3 ApproxNormal(V3 p) const {
+    const double kNRMin = 0., kNRMax = 1., kNSPhi = 2., kNEPhi = 3., kNSTheta = 4., kNETheta = 5.;
+    double side = 10.;
+    const double radius = Sqrt(p.Mag2());
+    const double rho = Sqrt(p.x * p.x + p.y * p.y);
+    double pPhi = ATan2(p.y, p.x);
+    const double pTheta = std::atan2(rho, p.z);
+    double distRMax, distRMin, distSPhi = 0., distEPhi = 0., distSTheta = 0., distETheta = 0., distMin = 0.;
+    double temp = radius - fRmax;
+    if (temp < 0.) temp = -1. * temp;
+    distRMax = temp;
+    temp = radius - fRmin;
+    if (temp < 0.) temp = -1. * temp;
+    distRMin = temp;
+    if (fRmin > 0. && distRMin < distRMax) {
+      distMin = distRMin;
+      side = kNRMin;
+    }
+    if (fRmin > 0. && !(distRMin < distRMax)) {
+      distMin = distRMax;
+      side = kNRMax;
+    }
+    if (!(fRmin > 0.)) {
+      distMin = distRMax;
+      side = kNRMax;
+    }
+    if (pPhi < 0.) pPhi = pPhi + (2 * kPi);
+    const bool phiCut = !IsFullPhiSphere() && rho > 0.;
+    temp = pPhi - (fSPhi + 2 * kPi);
+    if (temp < 0.) temp = -1. * temp;
+    if (phiCut && fSPhi < 0.) distSPhi = temp * rho;
+    temp = pPhi - fSPhi;
+    if (temp < 0.) temp = -1. * temp;
+    if (phiCut && !(fSPhi < 0.)) distSPhi = temp * rho;
+    temp = pPhi - fSPhi - fDPhi;
+    if (temp < 0.) temp = -1. * temp;
+    if (phiCut) distEPhi = temp * rho;
+    double prev = distMin;
+    if (phiCut && distSPhi < distEPhi && distSPhi < distMin) distMin = distSPhi;
+    if (phiCut && distSPhi < distEPhi && distSPhi < prev) side = kNSPhi;
+    prev = distMin;
+    if (phiCut && !(distSPhi < distEPhi) && distEPhi < distMin) distMin = distEPhi;
+    if (phiCut && !(distSPhi < distEPhi) && distEPhi < prev) side = kNEPhi;
+    const bool thetaCut = !IsFullThetaSphere() && radius > 0.;
+    temp = pTheta - fSTheta;
+    if (temp < 0.) temp = -1. * temp;
+    if (thetaCut) distSTheta = temp * radius;
+    temp = pTheta - fSTheta - fDTheta;
+    if (temp < 0.) temp = -1. * temp;
+    if (thetaCut) distETheta = temp * radius;
+    prev = distMin;
+    if (thetaCut && distSTheta < distETheta && distSTheta < distMin) distMin = distSTheta;
+    if (thetaCut && distSTheta < distETheta && distSTheta < prev) side = kNSTheta;
+    prev = distMin;
+    if (thetaCut && !(distSTheta < distETheta) && distETheta < distMin) distMin = distETheta;
+    if (thetaCut && !(distSTheta < distETheta) && distETheta < prev) side = kNETheta;
+    if (side == kNRMin) return V3{-p.x / radius, -p.y / radius, -p.z / radius};
+    if (side == kNRMax) return V3{p.x / radius, p.y / radius, p.z / radius};
+    if (side == kNSPhi) return V3{sinSPhi, -cosSPhi, 0};
+    if (side == kNEPhi) return V3{-sinEPhi, cosEPhi, 0};
+    if (side == kNSTheta) return V3{-cosSTheta * std::cos(pPhi), -cosSTheta * std::sin(pPhi), sinSTheta};
+    if (side == kNETheta) return V3{cosETheta * std::cos(pPhi), cosETheta * std::sin(pPhi), sinETheta};
+    return V3{0, 0, 0};
+  }
+
+  // SphereImplementation::NormalKernel, SphereImplementation.h:619-760: sum
+  // of the normals of every surface within tolerance (r shells 0.5e-9 mm,
+  // phi planes and theta cones half the angular tolerance), unit length when
+  // there are several; the approximate normal and invalid when there is none.
+  int Normal(V3 p, V3 &n) const override {
+    const double radius = Sqrt(p.Mag2());
+    const double rho = Sqrt(p.x * p.x + p.y * p.y);
+    double pPhi = ATan2(p.y, p.x);
+    const double pTheta = std::atan2(rho, p.z);
+    const double halfAng = 0.5 * kAngTol;
+    double distSPhi = kInfinity, distSTheta = kInfinity, distEPhi = kInfinity, distETheta = kInfinity;
+    V3 nR{0, 0, 0}, nPs{0, 0, 0}, nPe{0, 0, 0}, nTs{0, 0, 0}, nTe{0, 0, 0};
+    const V3 nZ{0., 0., 1.};
+    V3 norm{0, 0, 0}, sumnorm{0., 0., 0.};
+    const bool fullPhiSphere = IsFullPhiSphere();  // the local of that name in the reference
+    double temp = radius - fRmax;
+    if (temp < 0.) temp = -1. * temp;
+    const double distRMax = temp;
+    double distRMin = 0;
+    temp = radius - fRmin;
+    if (temp < 0.) temp = -1. * temp;
+    if (fRmin > 0) distRMin = temp;
+    if (rho > 0. && !IsFullSphere() && pPhi < fSPhi - halfAng) pPhi = pPhi + (2 * kPi);
+    if (rho > 0. && !IsFullSphere() && pPhi > ePhi + halfAng) pPhi = pPhi - (2 * kPi);
+    temp = pPhi - fSPhi;
+    if (temp < 0.) temp = -1. * temp;
+    if (!IsFullPhiSphere() && rho > 0.) distSPhi = temp;
+    temp = pPhi - ePhi;
+    if (temp < 0.) temp = -1. * temp;
+    if (!IsFullPhiSphere() && rho > 0.) distEPhi = temp;
+    if (!IsFullPhiSphere() && !fRmin) distSPhi = 0.;
+    if (!IsFullPhiSphere() && !fRmin) distEPhi = 0.;
+    if (!IsFullPhiSphere()) {
+      nPs = V3{sinSPhi, -cosSPhi, 0};
+      nPe = V3{-sinEPhi, cosEPhi, 0};
+    }
+    temp = pTheta - fSTheta;
+    if (temp < 0.) temp = -1. * temp;
+    if (!IsFullThetaSphere() && rho > 0.) distSTheta = temp;
+    temp = pTheta - eTheta;
+    if (temp < 0.) temp = -1. * temp;
+    if (!IsFullThetaSphere() && rho > 0.) distETheta = temp;
+    if (!IsFullThetaSphere() && rho > 0.) nTs = V3{-cosSTheta * p.x / rho, -cosSTheta * p.y / rho, sinSTheta};
+    if (!IsFullThetaSphere() && rho != 0.) nTe = V3{cosETheta * p.x / rho, cosETheta * p.y / rho, -sinETheta};
+    if (!IsFullThetaSphere() && !fRmin && fSTheta != 0.) {
+      distSTheta = 0.;
+      nTs = V3{0., 0., -1.};
+    }
+    if (!IsFullThetaSphere() && !fRmin && eTheta < kPi) {
+      distETheta = 0.;
+      nTe = V3{0., 0., 1.};
+    }
+    if (radius != 0.) nR = V3{p.x / radius, p.y / radius, p.z / radius};
+    double noSurfaces = 0;
+    const double halfCar = 0.5 * 1e-9;
+    if (distRMax <= halfCar) {
+      noSurfaces = noSurfaces + 1;
+      sumnorm = sumnorm + nR;
+    }
+    if (fRmin != 0. && distRMin <= halfCar) {
+      noSurfaces = noSurfaces + 1;
+      sumnorm = sumnorm - nR;
+    }
+    if (!IsFullPhiSphere() && distSPhi <= halfAng) {
+      noSurfaces = noSurfaces + 1;
+      sumnorm = sumnorm + nPs;
+    }
+    if (!IsFullPhiSphere() && distEPhi <= halfAng) {
+      noSurfaces = noSurfaces + 1;
+      sumnorm = sumnorm + nPe;
+    }
+    if (!IsFullThetaSphere() && distSTheta <= halfAng && fSTheta > 0.) {
+      noSurfaces = noSurfaces + 1;
+      if (radius <= halfCar && fullPhiSphere)
+        sumnorm = sumnorm + nZ;
+      else
+        sumnorm = sumnorm + nTs;
+    }
+    if (!IsFullThetaSphere() && distETheta <= halfAng && eTheta < kPi) {
+      noSurfaces = noSurfaces + 1;
+      if (radius <= halfCar && fullPhiSphere)
+        sumnorm = sumnorm - nZ;
+      else
+        sumnorm = sumnorm + nTe;
+      if (sumnorm.z == 0.) sumnorm = sumnorm + nZ;
+    }
+    if (noSurfaces == 0) norm = ApproxNormal(p);
+    if (noSurfaces == 1) norm = sumnorm;
+    if (!(noSurfaces == 1) && noSurfaces != 0) {
+      const double mag = Sqrt(sumnorm.Mag2());
+      norm = V3{sumnorm.x * 1. / mag, sumnorm.y * 1. / mag, sumnorm.z * 1. / mag};
+    }
+    n = norm;
+    return noSurfaces > 0 ? 1 : 0;
+  }
+
   // GenericKernelForContainsAndInside :801-893 (the body does not depend on
   // ForInside)
   void Kernel(V3 p, bool &cin, bool &cout) const {
@@ -417,6 +590,56 @@ struct Trapezoid : Shape {
     return true;
   }
   double Dist(int i, V3 p) const { return A[i] * p.x + B[i] * p.y + C[i] * p.z + D[i]; }
+
+  // UnplacedTrapezoid::ApproxSurfaceNormal, source/UnplacedTrapezoid.cpp:583-612
+  // (PlaneShell form): the nearest side plane's normal unless the z lid is
+  // at least as near.
+  V3 ApproxNormal(V3 p) const {
+    double safe = kInfinity;
+    int imin = 0;
+    for (int i = 0; i < 4; i++) {
+      const double dist = std::fabs(A[i] * p.x + B[i] * p.y + C[i] * p.z + D[i]);
+      if (dist < safe) {
+        safe = dist;
+        imin = i;
+      }
+    }
+    const double safez = std::fabs(std::fabs(p.z) - dz);
+    if (safe < safez) return V3{A[imin], B[imin], C[imin]};
+    return p.z > 0 ? V3{0, 0, 1} : V3{0, 0, -1};
+  }
+  // UnplacedTrapezoid::Normal, source/UnplacedTrapezoid.cpp:352-425
+  // (PlaneShell form, the reference's default build): sum of the side planes
+  // within kHalfTolerance and the z lid, as is with one surface, Unit() with
+  // several, the approximate normal (invalid) with none.
+  int Normal(V3 p, V3 &n) const override {
+    int noSurfaces = 0;
+    V3 sumnorm{0., 0., 0.};
+    for (int i = 0; i < 4; ++i) {
+      const double dist = A[i] * p.x + B[i] * p.y + C[i] * p.z + D[i];
+      if (std::fabs(dist) <= kHalfTolerance) {
+        noSurfaces++;
+        sumnorm = sumnorm + V3{A[i], B[i], C[i]};
+      }
+    }
+    const double distz = std::fabs(p.z) - dz;
+    if (std::fabs(distz) <= kHalfTolerance) {
+      noSurfaces++;
+      if (p.z >= 0.)
+        sumnorm = sumnorm + V3{0., 0., 1.};
+      else
+        sumnorm = sumnorm - V3{0., 0., 1.};
+    }
+    if (noSurfaces == 0) {
+      n = ApproxNormal(p);
+    } else if (noSurfaces == 1) {
+      n = sumnorm;
+    } else {
+      const double s = Sqrt(sumnorm.Mag2() + kMinimum);
+      n = V3{sumnorm.x / s, sumnorm.y / s, sumnorm.z / s};
+    }
+    return noSurfaces != 0 ? 1 : 0;
+  }
   // TrapezoidImplementation::GenericKernelForContainsAndInside :134-160 +
   // PlaneShell::GenericKernelForContainsAndInside (PlaneShell.h:128-152)
   template <bool ForInside>
