@@ -184,6 +184,32 @@ int vg_contains_local(const vg_geometry *g, int32_t placed, const double *x, con
 int vg_unplaced_contains(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z,
                          size_t n, uint8_t *out, void *stream);
 
+/* VPlacedVolume::Normal(localPoint, normal&) (volumes/PlacedVolume.h:266-272)
+ * in basket form, for the shapes whose reference overrides it: Box
+ * (PlacedBox.h:90, BoxImplementation.h:842-898), Tube (PlacedTube.h:94,
+ * source/UnplacedTube.cpp:141), Orb (PlacedOrb.h:95), Sphere
+ * (PlacedSphere.h:223), Cone (PlacedCone.h:96), Trd (PlacedTrd.h:96,
+ * source/UnplacedTrd.cpp:154), Trapezoid (PlacedTrapezoid.h:111,
+ * source/UnplacedTrapezoid.cpp:352) and Polycone (PlacedPolycone.h:87,
+ * source/UnplacedPolycone.cpp:684).  x,y,z are LOCAL points; nx,ny,nz get the
+ * normal as the reference computes it (also for points on no surface, where
+ * the reference returns a zero or approximate vector) and valid[i] its
+ * return value (1/0).  VG_ERR_UNSUPPORTED for the other shapes, whose
+ * reference Normal asserts. */
+int vg_normal(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z, size_t n,
+              double *nx, double *ny, double *nz, uint8_t *valid, void *stream);
+/* The USolids-compatible DistanceToOut(point, direction, normal&, convex&,
+ * stepMax) (volumes/ShapeImplementationHelper.h:277-295 and
+ * ScalarShapeImplementationHelper.h:249-264; volumes/USolidsInterfaceHelper.h
+ * declares it) in basket form on LOCAL rays: out[i] = DistanceToOut, the
+ * Normal of point + out[i] * direction in nx,ny,nz with its valid flag, and
+ * convex[i] = 1 (the reference's constant; convex may be NULL).  step_max
+ * may be NULL (+inf).  Same shapes as vg_normal. */
+int vg_distance_to_out_normal(const vg_geometry *g, int32_t placed, const double *x, const double *y,
+                              const double *z, const double *dx, const double *dy, const double *dz,
+                              const double *step_max, size_t n, double *out, double *nx, double *ny, double *nz,
+                              uint8_t *valid, uint8_t *convex, void *stream);
+
 /* VPlacedVolume::Contains(SOA3D, bool*)  (volumes/PlacedVolume.h:147) */
 int vg_contains(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z, size_t n,
                 uint8_t *out, void *stream);

@@ -50,6 +50,11 @@ class Lib:
         if self.has("basket"):
             f("basket").restype = ctypes.c_int
             f("basket").argtypes = f("query").argtypes
+        if self.has("normal"):
+            f("normal").restype = ctypes.c_int
+            f("normal").argtypes = [ctypes.c_void_p, ctypes.c_long] + [ctypes.c_void_p] * 7
+            f("distance_to_out_normal").restype = ctypes.c_int
+            f("distance_to_out_normal").argtypes = [ctypes.c_void_p, ctypes.c_long] + [ctypes.c_void_p] * 13
 
     def _f(self, name):
         return getattr(self.lib, self.prefix + name)
@@ -111,6 +116,42 @@ class Placed:
         if rc != 0:
             raise RuntimeError(f"{self.lib.prefix}basket failed rc={rc}")
         return out
+
+    def normal(self, p):
+        """VPlacedVolume::Normal on local points p (3, n): returns (normals
+        (3, n), valid uint8 (n,)).  ValueError where the reference has no
+        Normal for the shape."""
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        n = p.shape[1]
+        nrm = np.empty((3, n))
+        valid = np.empty(n, dtype=np.uint8)
+        rc = self.lib._f("normal")(self.h, n, _ptr(p[0]), _ptr(p[1]), _ptr(p[2]), _ptr(nrm[0]), _ptr(nrm[1]),
+                                   _ptr(nrm[2]), _ptr(valid))
+        if rc == -3:
+            raise ValueError(f"{self.lib.prefix}normal: the reference has no Normal for this shape")
+        if rc != 0:
+            raise RuntimeError(f"{self.lib.prefix}normal failed rc={rc}")
+        return nrm, valid
+
+    def distance_to_out_normal(self, p, d, step_max=None):
+        """The USolids DistanceToOut(point, direction, normal&, convex&,
+        stepMax) on local rays: (distance, normals (3, n), valid, convex)."""
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        n = p.shape[1]
+        dist = np.empty(n)
+        nrm = np.empty((3, n))
+        valid = np.empty(n, dtype=np.uint8)
+        convex = np.empty(n, dtype=np.uint8)
+        sm = None if step_max is None else np.ascontiguousarray(step_max, dtype=np.float64)
+        rc = self.lib._f("distance_to_out_normal")(self.h, n, _ptr(p[0]), _ptr(p[1]), _ptr(p[2]), _ptr(d[0]),
+                                                   _ptr(d[1]), _ptr(d[2]), _ptr(sm), _ptr(dist), _ptr(nrm[0]),
+                                                   _ptr(nrm[1]), _ptr(nrm[2]), _ptr(valid), _ptr(convex))
+        if rc == -3:
+            raise ValueError(f"{self.lib.prefix}distance_to_out_normal: the reference has no Normal for this shape")
+        if rc != 0:
+            raise RuntimeError(f"{self.lib.prefix}distance_to_out_normal failed rc={rc}")
+        return dist, nrm, valid, convex
 
     def __del__(self):
         try:

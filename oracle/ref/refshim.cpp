@@ -172,6 +172,52 @@ int vgref_query(void *h, int query, long n, const double *px, const double *py, 
   return 0;
 }
 
+// The shapes whose Normal the reference implements (PlacedBox.h:90,
+// PlacedTube.h:94, PlacedOrb.h:95, PlacedSphere.h:223, PlacedCone.h:96,
+// PlacedTrd.h:96, PlacedTrapezoid.h:111, PlacedPolycone.h:87); the others
+// assert (PlacedVolume.h:266-272, PlacedParaboloid.h:122).
+static bool HasNormal(int type) { return (type >= 1 && type <= 7) || type == 11; }
+
+// VPlacedVolume::Normal on local points.  The Cone's kernel asserts for a
+// point on no surface where NDEBUG is not defined (the strict build): such
+// points go only to the Release build.
+int vgref_normal(void *h, long n, const double *px, const double *py, const double *pz, double *nx, double *ny,
+                 double *nz, uint8_t *valid) {
+  auto *r = static_cast<RefPlaced *>(h);
+  if (!r) return -1;
+  if (!HasNormal(r->type)) return -3;
+  for (long i = 0; i < n; ++i) {
+    Vector3D<Precision> nv;
+    valid[i] = r->placed->Normal(V(px, py, pz, i), nv) ? 1 : 0;
+    nx[i] = nv.x(); ny[i] = nv.y(); nz[i] = nv.z();
+  }
+  return 0;
+}
+
+// The USolids DistanceToOut(point, direction, normal&, convex&, stepMax)
+// (volumes/ShapeImplementationHelper.h:277-295, compiled only with
+// VECGEOM_USOLIDS), evaluated with the reference's own virtuals and
+// Vector3D arithmetic: the distance, the Normal of point + d * direction,
+// convex = true.
+int vgref_distance_to_out_normal(void *h, long n, const double *px, const double *py, const double *pz,
+                                 const double *dx, const double *dy, const double *dz, const double *step_max,
+                                 double *dist, double *nx, double *ny, double *nz, uint8_t *valid, uint8_t *convex) {
+  auto *r = static_cast<RefPlaced *>(h);
+  if (!r) return -1;
+  if (!HasNormal(r->type)) return -3;
+  for (long i = 0; i < n; ++i) {
+    const Vector3D<Precision> point = V(px, py, pz, i), direction = V(dx, dy, dz, i);
+    const double d = r->placed->DistanceToOut(point, direction, step_max ? step_max[i] : kInfinity);
+    Vector3D<double> hitpoint = point + d * direction;
+    Vector3D<Precision> nv;
+    valid[i] = r->placed->Normal(hitpoint, nv) ? 1 : 0;
+    dist[i] = d;
+    nx[i] = nv.x(); ny[i] = nv.y(); nz[i] = nv.z();
+    convex[i] = 1;
+  }
+  return 0;
+}
+
 // The reference's own SOA3D basket virtuals (volumes/PlacedVolume.h:147-246,
 // basket loops volumes/ShapeImplementationHelper.h:588-748), each thread on a
 // contiguous range through non-owning SOA3D views (base/SOA3D.h:173-176).

@@ -137,6 +137,45 @@ int vgo_query(void *h, int query, long n, const double *px, const double *py, co
   return -2;
 }
 
+// VPlacedVolume::Normal on local points: nx, ny, nz and valid (1/0) per
+// point.  Returns -3 when the shape has no Normal in the reference.
+int vgo_normal(void *h, long n, const double *px, const double *py, const double *pz, double *nx, double *ny,
+               double *nz, uint8_t *valid) {
+  auto *o = static_cast<PlacedOwner *>(h);
+  if (!o) return -1;
+  V3 t;
+  if (o->shape->Normal(V3{0, 0, 0}, t) < 0) return -3;
+  for (long i = 0; i < n; ++i) {
+    V3 v;
+    valid[i] = (uint8_t)o->shape->Normal(V3{px[i], py[i], pz[i]}, v);
+    nx[i] = v.x; ny[i] = v.y; nz[i] = v.z;
+  }
+  return 0;
+}
+
+// The USolids DistanceToOut(point, direction, normal&, convex&, stepMax) on
+// local rays: the distance, the normal at the exit point, its valid flag and
+// convex (always 1).  Returns -3 when the shape has no Normal.
+int vgo_distance_to_out_normal(void *h, long n, const double *px, const double *py, const double *pz,
+                               const double *dx, const double *dy, const double *dz, const double *step_max,
+                               double *dist, double *nx, double *ny, double *nz, uint8_t *valid, uint8_t *convex) {
+  auto *o = static_cast<PlacedOwner *>(h);
+  if (!o) return -1;
+  V3 t;
+  if (o->shape->Normal(V3{0, 0, 0}, t) < 0) return -3;
+  for (long i = 0; i < n; ++i) {
+    V3 v;
+    bool cv;
+    int ok;
+    dist[i] = DistanceToOutNormal(*o->shape, V3{px[i], py[i], pz[i]}, V3{dx[i], dy[i], dz[i]},
+                                  step_max ? step_max[i] : kInfinity, v, cv, ok);
+    nx[i] = v.x; ny[i] = v.y; nz[i] = v.z;
+    valid[i] = (uint8_t)ok;
+    convex[i] = cv ? 1 : 0;
+  }
+  return 0;
+}
+
 // Forward transform of points by a placement (Transform<t,r> with the
 // placement's codes); used by tests to build local-frame inputs.
 void vgo_transform_points(const double *xf12, long n, const double *x, const double *y, const double *z, double *ox,

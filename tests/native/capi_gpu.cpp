@@ -105,6 +105,30 @@ int main() {
   hc = Host(c, 3);
   Expect(hc[0] == 1 && hc[1] == 0 && hc[2] == 0, "UnplacedContains of the local points");
 
+  // Normal and the USolids DistanceToOut with the exit normal, on local points
+  // of the box (half-lengths 10, 15, 20): the +x face, an edge, the centre.
+  {
+    const std::vector<double> nx0 = {10, -10, 0}, ny0 = {0, 15, 0}, nz0 = {3, 0, 0};
+    double *qx = Dev(nx0), *qy = Dev(ny0), *qz = Dev(nz0);
+    double *ox = Dev(std::vector<double>(3, 0.)), *oy = Dev(std::vector<double>(3, 0.)),
+           *oz = Dev(std::vector<double>(3, 0.));
+    uint8_t *ok = Dev(std::vector<uint8_t>(3, 9));
+    Ok(vg_normal(g, 1, qx, qy, qz, 3, ox, oy, oz, ok, nullptr), "vg_normal");
+    auto hv = Host(ok, 3);
+    auto hx = Host(ox, 3), hy = Host(oy, 3);
+    Expect(hv[0] == 1 && hv[1] == 1 && hv[2] == 0 && hx[0] == 1. && Near(hx[1], -M_SQRT1_2) && Near(hy[1], M_SQRT1_2),
+           "box Normal: +x face, (-x, +y) edge, centre invalid");
+    Ok(vg_distance_to_out_normal(g, 1, lx, ly, lz, ddx, ddy, ddz, nullptr, 1, out, ox, oy, oz, ok, c, nullptr),
+       "vg_distance_to_out_normal");
+    auto hdo = Host(out, 1);
+    hx = Host(ox, 1);
+    hv = Host(ok, 1);
+    auto hcv = Host(c, 1);
+    Expect(Near(hdo[0], 10.) && hx[0] == 1. && hv[0] == 1 && hcv[0] == 1,
+           "box DistanceToOut from its centre along +x = 10, exit normal +x, convex");
+    Expect(vg_normal(g, 3, qx, qy, qz, 3, ox, oy, oz, ok, nullptr) == VG_OK, "orb Normal");
+  }
+
   // Scalar virtuals.
   const double p1[3] = {35, 1, 2}, up[3] = {0, 1, 0};
   double r = -1, loc[3] = {0, 0, 0};

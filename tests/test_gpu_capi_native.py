@@ -1,6 +1,7 @@
 """The C-ABI called from C++ (tests/native/capi_gpu.cpp, linked against
 vecgeom_amd/lib/libvecgeom_amd.so, no Python in between): basket and scalar
-shape queries, Contains(point, local&), UnplacedContains, the
+shape queries, Contains(point, local&), UnplacedContains, Normal and the
+USolids DistanceToOut with the exit normal, the
 no-allocation batch and the navigator on a small geometry with closed-form
 answers."""
 import os

@@ -63,6 +63,8 @@ _SIGS = {
     "vg_safety_to_out_minimize": [c_vp, c_i32, c_vp, c_vp, c_vp, c_sz, c_vp, c_vp],
     "vg_contains_local": [c_vp, c_i32, c_vp, c_vp, c_vp, c_sz, c_vp, c_vp, c_vp, c_vp, c_vp],
     "vg_unplaced_contains": [c_vp, c_i32, c_vp, c_vp, c_vp, c_sz, c_vp, c_vp],
+    "vg_normal": [c_vp, c_i32, c_vp, c_vp, c_vp, c_sz, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "vg_distance_to_out_normal": [c_vp, c_i32] + [c_vp] * 7 + [c_sz] + [c_vp] * 7,
     "vg_scalar_query": [c_vp, c_i32, c_i32, c_vp, c_vp, ctypes.c_double, c_vp, c_vp],
 }
 _RESTYPE = {"vg_last_error": ctypes.c_char_p, "vg_version": ctypes.c_char_p}

@@ -205,6 +205,35 @@ int vg_unplaced_contains(const vg_geometry *g, int32_t placed, const double *x, 
                              static_cast<hipStream_t>(stream));
 }
 
+int vg_normal(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z, size_t n,
+              double *nx, double *ny, double *nz, uint8_t *valid, void *stream) {
+  if (!ValidPlaced(g, placed)) return SetError(VG_ERR_INVALID, "bad geometry handle or placed id");
+  if (n == 0) return VG_OK;
+  if (!x || !y || !z || !nx || !ny || !nz || !valid) return SetError(VG_ERR_INVALID, "null point or output array");
+  if (!ShapeHasNormal(g->placed[placed].type))
+    return SetError(VG_ERR_UNSUPPORTED, "the reference has no Normal for this shape type");
+  int rc = EnsureDevice();
+  if (rc) return rc;
+  return LaunchNormal(g, placed, x, y, z, nullptr, nullptr, nullptr, nullptr, n, nullptr, nx, ny, nz, valid, nullptr,
+                      static_cast<hipStream_t>(stream));
+}
+
+int vg_distance_to_out_normal(const vg_geometry *g, int32_t placed, const double *x, const double *y,
+                              const double *z, const double *dx, const double *dy, const double *dz,
+                              const double *step_max, size_t n, double *out, double *nx, double *ny, double *nz,
+                              uint8_t *valid, uint8_t *convex, void *stream) {
+  if (!ValidPlaced(g, placed)) return SetError(VG_ERR_INVALID, "bad geometry handle or placed id");
+  if (n == 0) return VG_OK;
+  if (!x || !y || !z || !dx || !dy || !dz || !out || !nx || !ny || !nz || !valid)
+    return SetError(VG_ERR_INVALID, "null point, direction or output array");
+  if (!ShapeHasNormal(g->placed[placed].type))
+    return SetError(VG_ERR_UNSUPPORTED, "the reference has no Normal for this shape type");
+  int rc = EnsureDevice();
+  if (rc) return rc;
+  return LaunchNormal(g, placed, x, y, z, dx, dy, dz, step_max, n, out, nx, ny, nz, valid, convex,
+                      static_cast<hipStream_t>(stream));
+}
+
 int vg_scalar_query(vg_geometry *g, int32_t placed, int32_t query, const double point[3], const double dir[3],
                     double step_max, double *result, double local[3]) {
   if (!ValidPlaced(g, placed)) return SetError(VG_ERR_INVALID, "bad geometry handle or placed id");

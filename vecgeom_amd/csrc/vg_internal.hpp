@@ -126,6 +126,13 @@ int LaunchContainsExtra(const vg_geometry *g, int32_t placed, const double *x, c
 int LaunchQuery(const vg_geometry *g, int32_t placed, int q, const QArgs &a, hipStream_t s);
 // Basket queries on a Boolean volume (vg_boolean.hip).
 int LaunchBooleanQuery(const vg_geometry *g, int32_t placed, int q, const QArgs &a, hipStream_t s);
+// Normal baskets (vg_normal.hip): with dist == nullptr the Normal of local
+// points; otherwise the USolids DistanceToOut with the normal at the exit.
+int LaunchNormal(const vg_geometry *g, int32_t placed, const double *x, const double *y, const double *z,
+                 const double *dx, const double *dy, const double *dz, const double *step_max, uint64_t n,
+                 double *dist, double *nx, double *ny, double *nz, uint8_t *valid, uint8_t *convex, hipStream_t s);
+// Shape types whose reference implements VPlacedVolume::Normal.
+bool ShapeHasNormal(int type);
 // Placement kind (identity / translation / rotation / general) of 12
 // placement doubles, as Transformation3D::SetProperties decides it.
 int PlacementKind(const double *xf12);

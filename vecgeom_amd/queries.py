@@ -77,6 +77,36 @@ def unplaced_contains(geom, placed, lp, out=None):
     return out
 
 
+def normal(geom, placed, lp):
+    """VPlacedVolume::Normal(localPoint, normal&) (volumes/PlacedVolume.h:266-272
+    and the shapes' overrides) on local points: returns ((nx, ny, nz) float64,
+    uint8 valid).  Box, Tube, Orb, Sphere, Cone, Trd, Trapezoid and Polycone;
+    the other shapes raise (their reference Normal asserts)."""
+    n = _check_soa(lp, "local points")
+    dev = lp[0].device
+    nrm = tuple(torch.empty(n, dtype=torch.float64, device=dev) for _ in range(3))
+    valid = torch.empty(n, dtype=torch.uint8, device=dev)
+    capi.call("vg_normal", geom.handle, int(placed), *_ptrs(lp), n, *_ptrs(nrm), valid.data_ptr(), _stream())
+    return nrm, valid
+
+
+def distance_to_out_normal(geom, placed, lp, ld, step_max=None):
+    """The USolids DistanceToOut(point, direction, normal&, convex&, stepMax)
+    (volumes/ShapeImplementationHelper.h:277-295) on local rays: returns
+    (distance, (nx, ny, nz), uint8 valid, uint8 convex)."""
+    n = _check_soa(lp, "local points")
+    _check_soa(ld, "local directions", n, lp[0].device)
+    dev = lp[0].device
+    out = torch.empty(n, dtype=torch.float64, device=dev)
+    nrm = tuple(torch.empty(n, dtype=torch.float64, device=dev) for _ in range(3))
+    valid = torch.empty(n, dtype=torch.uint8, device=dev)
+    convex = torch.empty(n, dtype=torch.uint8, device=dev)
+    capi.call("vg_distance_to_out_normal", geom.handle, int(placed), *_ptrs(lp), *_ptrs(ld),
+              _step_max(step_max, n, dev), n, out.data_ptr(), *_ptrs(nrm), valid.data_ptr(), convex.data_ptr(),
+              _stream())
+    return out, nrm, valid, convex
+
+
 CONTAINS_LOCAL, UNPLACED_CONTAINS = 7, 8
 
 
