@@ -121,6 +121,7 @@ def parse():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision workload and forms")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage batched form")
     ap.add_argument("--no-boolean", action="store_true", help="skip the Boolean-volume queries")
+    ap.add_argument("--no-normal", action="store_true", help="skip the surface-normal queries")
     ap.add_argument("--nav-tracks", type=int, default=1 << 20, help="tracks of the nested-geometry navigation run")
     ap.add_argument("--nav-steps", type=int, default=3, help="timed trace launches")
     ap.add_argument("--no-nav", action="store_true")
@@ -405,6 +406,60 @@ def boolean_bench(args, dev, stream, with_cpu):
     tot_us = sum(r["avg_us"] for r in rows)
     return {"workload": "Boolean volumes (union, intersection, subtraction, hole, Polyhedron intersection), "
                         "random placements, 2^20 points per volume, six queries, one launch per (volume, query)",
+            "value": round(tot_q / tot_us / 1e3, 3), "unit": "Gqueries/s", "per_kernel": rows}
+
+
+def normal_bench(args, dev, stream):
+    """Surface normals (vg_normal, vg_distance_to_out_normal) on the zoo shapes
+    whose reference has a Normal: per shape, 2^22 interior points (uniform in
+    the extent, kept where the device Contains holds) with isotropic
+    directions; DistanceToOut with the exit normal on those rays, then Normal
+    on the exit points (surface points).  Per-launch kernels timed with HIP
+    events, mean of the timed launches; algorithmic bytes: Normal 49 B per
+    query (24 in, 3 doubles and a flag out), DistanceToOut with normal 82 B
+    (48 in, a distance, 3 doubles and two flags out)."""
+    import torch
+
+    from vecgeom_amd import geometry as G
+    from vecgeom_amd import queries as Q
+    from vecgeom_amd import workloads as W
+
+    n = 1 << 22
+    shapes = [(nm, u) for nm, u in W.zoo_shapes() if u.type in (G.VG_BOX, G.VG_TUBE, G.VG_ORB, G.VG_SPHERE, G.VG_CONE,
+                                                                  G.VG_TRD, G.VG_TRAPEZOID, G.VG_POLYCONE)]
+    I = G.Transformation3D()
+    geom, ids, _ = G.single_shape_geometry([u for _, u in shapes], [I] * len(shapes))
+    rows = []
+    for (name, u), pid in zip(shapes, ids):
+        rng = np.random.Generator(np.random.Philox(W.SEED_BASE + 900 + pid))
+        lo, hi = geom.placed_extent(pid)  # identity placement: the local extent
+        kept = []
+        while sum(k.shape[1] for k in kept) < n:
+            cand = rng.uniform(np.asarray(lo), np.asarray(hi), size=(n, 3)).T.copy()
+            ct = tuple(torch.from_numpy(cand[i]).to(dev) for i in range(3))
+            inside = Q.contains(geom, pid, ct).cpu().numpy().astype(bool)
+            kept.append(cand[:, inside])
+        p = np.concatenate(kept, axis=1)[:, :n]
+        d = W.isotropic(n, rng)
+        pt = tuple(torch.from_numpy(np.ascontiguousarray(p[i])).to(dev) for i in range(3))
+        dt = tuple(torch.from_numpy(np.ascontiguousarray(d[i])).to(dev) for i in range(3))
+        dist, _, _, _ = Q.distance_to_out_normal(geom, pid, pt, dt)
+        hit = tuple((pt[i] + dt[i] * dist).contiguous() for i in range(3))
+        forms = [("normal", lambda: Q.normal(geom, pid, hit), 49),
+                 ("dist_out_normal", lambda: Q.distance_to_out_normal(geom, pid, pt, dt), 82)]
+        for qn, fn, bpq in forms:
+            for _ in range(2):
+                fn()
+            _, ms = time_steps([((name, qn), fn)], 5, stream)
+            us = 1e3 * float(np.mean(ms[0]))
+            rows.append({"shape": name, "query": qn, "n": n, "avg_us": round(us, 3),
+                         "gqps": round(n / us / 1e3, 3), "gbs": round(n * bpq / us / 1e3, 1)})
+    geom.free()
+    tot_q = sum(r["n"] for r in rows)
+    tot_us = sum(r["avg_us"] for r in rows)
+    return {"workload": "Normal on 2^22 exit points and DistanceToOut with the exit normal on 2^22 interior rays "
+                        "per shape (Box, Orb, Sphere, Tube, Cone, Trd, Trapezoid, Polycone), one launch per "
+                        "(shape, query)",
             "value": round(tot_q / tot_us / 1e3, 3), "unit": "Gqueries/s", "per_kernel": rows}
 
 
@@ -1124,6 +1179,11 @@ def main():
         log("Boolean volumes")
         boolean = boolean_bench(args, dev, stream, not args.no_cpu)
 
+    normals = None
+    if not args.no_normal:
+        log("surface normals")
+        normals = normal_bench(args, dev, stream)
+
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gqueries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
@@ -1144,6 +1204,7 @@ def main():
         "nav": nav,
         "serve": serve,
         "boolean": boolean,
+        "normals": normals,
         "per_kernel": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()} for r in rows],
     }
     print(json.dumps(line), flush=True)
