@@ -409,7 +409,7 @@ def boolean_bench(args, dev, stream, with_cpu):
             "value": round(tot_q / tot_us / 1e3, 3), "unit": "Gqueries/s", "per_kernel": rows}
 
 
-def normal_bench(args, dev, stream):
+def normal_bench(args, dev, stream, with_cpu):
     """Surface normals (vg_normal, vg_distance_to_out_normal) on the zoo shapes
     whose reference has a Normal: per shape, 2^22 interior points (uniform in
     the extent, kept where the device Contains holds) with isotropic
@@ -417,7 +417,9 @@ def normal_bench(args, dev, stream):
     on the exit points (surface points).  Per-launch kernels timed with HIP
     events, mean of the timed launches; algorithmic bytes: Normal 49 B per
     query (24 in, 3 doubles and a flag out), DistanceToOut with normal 82 B
-    (48 in, a distance, 3 doubles and two flags out)."""
+    (48 in, a distance, 3 doubles and two flags out).  With the CPU leg: the
+    reference's own Normal and DistanceToOut-then-Normal (its Release build,
+    oracle/_ref) on one host thread over every 64th point and ray."""
     import torch
 
     from vecgeom_amd import geometry as G
@@ -430,6 +432,11 @@ def normal_bench(args, dev, stream):
     I = G.Transformation3D()
     geom, ids, _ = G.single_shape_geometry([u for _, u in shapes], [I] * len(shapes))
     rows = []
+    ref = None
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        ref = po.reference(fast=True)
     for (name, u), pid in zip(shapes, ids):
         rng = np.random.Generator(np.random.Philox(W.SEED_BASE + 900 + pid))
         lo, hi = geom.placed_extent(pid)  # identity placement: the local extent
@@ -454,6 +461,16 @@ def normal_bench(args, dev, stream):
             us = 1e3 * float(np.mean(ms[0]))
             rows.append({"shape": name, "query": qn, "n": n, "avg_us": round(us, 3),
                          "gqps": round(n / us / 1e3, 3), "gbs": round(n * bpq / us / 1e3, 1)})
+        if ref is not None:
+            h = ref.placed(u.type, u.params, I.array())
+            sel = np.arange(0, n, 64)
+            hp = np.stack([a.cpu().numpy()[sel] for a in hit])
+            t0 = time.perf_counter()
+            h.normal(hp)
+            rows[-2]["cpu_ref_1thread_gqps"] = round(len(sel) / (time.perf_counter() - t0) / 1e9, 5)
+            t0 = time.perf_counter()
+            h.distance_to_out_normal(p[:, sel], d[:, sel])
+            rows[-1]["cpu_ref_1thread_gqps"] = round(len(sel) / (time.perf_counter() - t0) / 1e9, 5)
     geom.free()
     tot_q = sum(r["n"] for r in rows)
     tot_us = sum(r["avg_us"] for r in rows)
@@ -1182,7 +1199,7 @@ def main():
     normals = None
     if not args.no_normal:
         log("surface normals")
-        normals = normal_bench(args, dev, stream)
+        normals = normal_bench(args, dev, stream, not args.no_cpu)
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gqueries/s", "n_gpus": 1, "steps": args.steps,
