@@ -27,6 +27,8 @@ CALLS = [
     ("placed_distance_to_out", lambda p, d: Q.placed_distance_to_out(_Geom(), 0, p, d)),
     ("safety_to_in", lambda p, d: Q.safety_to_in(_Geom(), 0, p)),
     ("safety_to_out", lambda p, d: Q.safety_to_out(_Geom(), 0, p)),
+    ("normal", lambda p, d: Q.normal(_Geom(), 0, p)),
+    ("distance_to_out_normal", lambda p, d: Q.distance_to_out_normal(_Geom(), 0, p, d)),
 ]
 
 
