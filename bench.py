@@ -348,7 +348,7 @@ def boolean_bench(args, dev, stream, with_cpu):
     """Boolean volumes (union, intersection, subtraction; see
     workloads.boolean_shapes): each query on 2^20 points uniform in a 120 mm
     cube around the volume, random placement, per-launch kernels timed with
-    HIP events (mean of the timed launches); Contains / Inside / SafetyToIn /
+    HIP events (median of five launches); Contains / Inside / SafetyToIn /
     DistanceToIn on every point, DistanceToOut / SafetyToOut on the points
     inside.  With the CPU leg: the reference's scalar virtuals on one host
     thread over a strided sample of the same queries."""
@@ -389,7 +389,7 @@ def boolean_bench(args, dev, stream, with_cpu):
             for _ in range(2):
                 fn()
             _, ms = time_steps([((name, qn), fn)], 5, stream)
-            us = 1e3 * float(np.mean(ms[0]))
+            us = 1e3 * float(np.median(ms[0]))  # the first run holds the host's start-up gap
             rows.append({"shape": name, "query": qn, "n": m, "avg_us": round(us, 3),
                          "gqps": round(m / us / 1e3, 3)})
         if ref is not None:
@@ -415,7 +415,7 @@ def normal_bench(args, dev, stream, with_cpu):
     the extent, kept where the device Contains holds) with isotropic
     directions; DistanceToOut with the exit normal on those rays, then Normal
     on the exit points (surface points).  Per-launch kernels timed with HIP
-    events, mean of the timed launches; algorithmic bytes: Normal 49 B per
+    events, median of five launches; algorithmic bytes: Normal 49 B per
     query (24 in, 3 doubles and a flag out), DistanceToOut with normal 82 B
     (48 in, a distance, 3 doubles and two flags out).  With the CPU leg: the
     reference's own Normal and DistanceToOut-then-Normal (its Release build,
@@ -458,7 +458,7 @@ def normal_bench(args, dev, stream, with_cpu):
             for _ in range(2):
                 fn()
             _, ms = time_steps([((name, qn), fn)], 5, stream)
-            us = 1e3 * float(np.mean(ms[0]))
+            us = 1e3 * float(np.median(ms[0]))  # the first run holds the host's start-up gap
             rows.append({"shape": name, "query": qn, "n": n, "avg_us": round(us, 3),
                          "gqps": round(n / us / 1e3, 3), "gbs": round(n * bpq / us / 1e3, 1)})
         if ref is not None:
@@ -1094,13 +1094,15 @@ def main():
     torch.cuda.synchronize()
 
     # Census (untimed): HIP events around every launch give the per-kernel
-    # breakdown and pick the dominant kernel.
-    _, cen = time_steps(launches, 3, stream)
+    # breakdown and pick the dominant kernel.  The median over five steps:
+    # the first launch of the first step also holds the host's gap after the
+    # synchronisation that starts the run.
+    _, cen = time_steps(launches, 5, stream)
     keys = [key for key, _ in launches]
     count = {(it["name"], q): c for it in wl.items for q, c in it["count"].items()}
     rows = []
     for k, key in enumerate(keys):
-        ms = float(np.mean(cen[k]))
+        ms = float(np.median(cen[k]))
         cnt = count[key]
         rows.append({"shape": key[0], "query": key[1], "n": cnt, "avg_us": 1e3 * ms,
                      "gqps": cnt / (ms * 1e-3) / 1e9 if ms > 0 else None,

@@ -371,6 +371,7 @@ __global__ __launch_bounds__(kBlock) void PlainQueryKernel(const PlacedRec *__re
 }
 
 bool Aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+bool Aligned32(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 31u) == 0; }
 
 int g_variant = 0;  // 0 = tuned kernels, 1 = plain FP64 form
 
@@ -402,6 +403,17 @@ int LaunchQ(const typename S::P &P, const Xf &X, const QArgs &a, int stage_len, 
       const uint64_t blocks = (a.n + kBlock - 1) / kBlock;
       hipLaunchKernelGGL((ShapeQueryKernel<S, XK, Q, 1, true>), dim3((unsigned)blocks), dim3(kBlock),
                          stage_len * sizeof(double), s, P, X, a, stage_len);
+      return CheckHip(hipGetLastError(), "shape query launch");
+    }
+  }
+  if constexpr (Q == kQContains) {
+    // Four queries per lane: the flags leave as one 4-byte store per lane
+    // (two queries per lane would store 2 bytes).
+    if (Aligned32(a.x) && Aligned32(a.y) && Aligned32(a.z) && (reinterpret_cast<uintptr_t>(a.out) & 3u) == 0) {
+      constexpr int IT = 4;
+      const uint64_t blocks = (a.n + (uint64_t)kBlock * IT - 1) / ((uint64_t)kBlock * IT);
+      hipLaunchKernelGGL((ShapeQueryKernel<S, XK, Q, IT, false>), dim3((unsigned)blocks), dim3(kBlock), 0, s, P, X, a,
+                         0);
       return CheckHip(hipGetLastError(), "shape query launch");
     }
   }
