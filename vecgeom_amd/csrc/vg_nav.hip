@@ -156,7 +156,14 @@ struct TraceArgs {
   double pstep;
   int32_t kind;
   int32_t max_steps;
+  int32_t stage_boxes;  // 1: every daughter slot's box is copied to LDS first
+  int32_t pad;
 };
+
+// Daughter-slot boxes (8 floats each) are staged in LDS by the trace kernel
+// when they take at most this many bytes: 20 resident workgroups per CU
+// (five waves per SIMD, two waves a workgroup) then still fit in 160 KiB.
+constexpr size_t kNavBoxLds = 8192;
 
 // Occupancy floor of the fused trace kernel over the basic shape set: it is
 // latency-bound, and 5 waves per SIMD (instead of the 3 its 145 VGPRs allow,
@@ -168,6 +175,14 @@ constexpr int TraceWaves(uint32_t m) { return m == kBasicShapes ? 5 : 1; }
 template <int D, uint32_t M, bool MX>
 __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(TraceWaves(M)))) void TraceKernel(
     const NavDev g, const TraceArgs a) {
+  extern __shared__ float nav_boxes[];
+  NavDev gl = g;
+  if (a.stage_boxes) {  // uniform over the grid
+    const int nf = 8 * g.n_daughters;
+    for (int j = threadIdx.x; j < nf; j += kNavBlock) nav_boxes[j] = g.slot_box[j];
+    __syncthreads();
+    gl.slot_box = nav_boxes;
+  }
   const uint64_t i = (uint64_t)blockIdx.x * kNavBlock + threadIdx.x;
   if (i >= a.n) return;
   V3 p{a.x[i], a.y[i], a.z[i]};
@@ -175,7 +190,7 @@ __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(Trace
   double dist;
   uint64_t h;
   int nexact;
-  const int k = TraceOne<D, false, M, MX>(g, p, d, a.pstep, a.kind, a.max_steps, dist, h, nexact);
+  const int k = TraceOne<D, false, M, MX>(gl, p, d, a.pstep, a.kind, a.max_steps, dist, h, nexact);
   a.nsteps[i] = k;
   a.fx[i] = p.x;
   a.fy[i] = p.y;
@@ -233,8 +248,11 @@ int NavTrace(const vg_geometry *g, int kind, const double *x, const double *y, c
              const double *dy, const double *dz, uint64_t n, double pstep, int max_steps, int32_t *nsteps, double *fx,
              double *fy, double *fz, double *travelled, uint64_t *hash, int32_t *exact, hipStream_t st) {
   if (n == 0) return VG_OK;
-  TraceArgs a{x, y, z, dx, dy, dz, nsteps, fx, fy, fz, travelled, hash, exact, n, pstep, kind, max_steps};
-  VGA_NAV_VARIANTS_MX(g, kind == VG_NAV_ABBOX_MIXED, TraceKernel, dim3(Blocks(n)), dim3(kNavBlock), 0, st, Dev(g), a);
+  TraceArgs a{x, y, z, dx, dy, dz, nsteps, fx, fy, fz, travelled, hash, exact, n, pstep, kind, max_steps, 0, 0};
+  const size_t box_bytes = 8 * sizeof(float) * g->daughters.size();
+  a.stage_boxes = (box_bytes > 0 && box_bytes <= kNavBoxLds) ? 1 : 0;
+  VGA_NAV_VARIANTS_MX(g, kind == VG_NAV_ABBOX_MIXED, TraceKernel, dim3(Blocks(n)), dim3(kNavBlock),
+                      a.stage_boxes ? box_bytes : 0, st, Dev(g), a);
   return CheckHip(hipGetLastError(), "trace launch");
 }
 
