@@ -156,13 +156,13 @@ struct TraceArgs {
   double pstep;
   int32_t kind;
   int32_t max_steps;
-  int32_t stage_boxes;  // 1: every daughter slot's box is copied to LDS first
-  int32_t pad;
+  int32_t stage_lo, stage_hi;  // daughter slots copied to LDS first (none when equal)
 };
 
-// Daughter-slot boxes (8 floats each) are staged in LDS by the trace kernel
-// when they take at most this many bytes: 20 resident workgroups per CU
-// (five waves per SIMD, two waves a workgroup) then still fit in 160 KiB.
+// The world volume's daughter-slot boxes (8 floats each) are staged in LDS
+// by the trace kernel when they take at most this many bytes: 20 resident
+// workgroups per CU (five waves per SIMD, two waves a workgroup) then still
+// fit in 160 KiB.
 constexpr size_t kNavBoxLds = 8192;
 
 // Occupancy floor of the fused trace kernel over the basic shape set: it is
@@ -177,11 +177,14 @@ __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(Trace
     const NavDev g, const TraceArgs a) {
   extern __shared__ float nav_boxes[];
   NavDev gl = g;
-  if (a.stage_boxes) {  // uniform over the grid
-    const int nf = 8 * g.n_daughters;
-    for (int j = threadIdx.x; j < nf; j += kNavBlock) nav_boxes[j] = g.slot_box[j];
+  if (a.stage_hi > a.stage_lo) {  // uniform over the grid
+    const int nf = 8 * (a.stage_hi - a.stage_lo);
+    const float *src = g.slot_box + 8 * (size_t)a.stage_lo;
+    for (int j = threadIdx.x; j < nf; j += kNavBlock) nav_boxes[j] = src[j];
     __syncthreads();
-    gl.slot_box = nav_boxes;
+    gl.lds_box = nav_boxes;
+    gl.lds_lo = a.stage_lo;
+    gl.lds_hi = a.stage_hi;
   }
   const uint64_t i = (uint64_t)blockIdx.x * kNavBlock + threadIdx.x;
   if (i >= a.n) return;
@@ -249,10 +252,14 @@ int NavTrace(const vg_geometry *g, int kind, const double *x, const double *y, c
              double *fy, double *fz, double *travelled, uint64_t *hash, int32_t *exact, hipStream_t st) {
   if (n == 0) return VG_OK;
   TraceArgs a{x, y, z, dx, dy, dz, nsteps, fx, fy, fz, travelled, hash, exact, n, pstep, kind, max_steps, 0, 0};
-  const size_t box_bytes = 8 * sizeof(float) * g->daughters.size();
-  a.stage_boxes = (box_bytes > 0 && box_bytes <= kNavBoxLds) ? 1 : 0;
+  const PlacedRec *w = g->world >= 0 ? &g->placed[g->world] : nullptr;
+  const size_t box_bytes = w ? 8 * sizeof(float) * (size_t)(w->dend - w->dbegin) : 0;
+  if (box_bytes > 0 && box_bytes <= kNavBoxLds) {
+    a.stage_lo = w->dbegin;
+    a.stage_hi = w->dend;
+  }
   VGA_NAV_VARIANTS_MX(g, kind == VG_NAV_ABBOX_MIXED, TraceKernel, dim3(Blocks(n)), dim3(kNavBlock),
-                      a.stage_boxes ? box_bytes : 0, st, Dev(g), a);
+                      a.stage_hi > a.stage_lo ? box_bytes : 0, st, Dev(g), a);
   return CheckHip(hipGetLastError(), "trace launch");
 }
 

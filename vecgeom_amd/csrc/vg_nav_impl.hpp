@@ -44,6 +44,11 @@ struct NavDev {
   int32_t n_placed;
   int32_t n_daughters;
   int32_t pad;
+  // Slots [lds_lo, lds_hi) -- the world volume's daughters -- read from a
+  // copy in LDS made by the trace kernel (none elsewhere).
+  const float *lds_box = nullptr;
+  int32_t lds_lo = 0;
+  int32_t lds_hi = 0;
   VGA_NAV const PlacedRec &Pv(int id) const {
     VGA_NAV_CHECK(id >= 0 && id < n_placed, "placed id out of range");
     return placed[id];
@@ -58,6 +63,7 @@ struct NavDev {
   }
   VGA_NAV const float *SlotBox(int k) const {
     VGA_NAV_CHECK(k >= 0 && k < n_daughters, "box slot out of range");
+    if (k >= lds_lo && k < lds_hi) return lds_box + 8 * (k - lds_lo);
     return slot_box + 8 * (size_t)k;
   }
   VGA_NAV const B32 &Bound(int id) const {
