@@ -48,6 +48,9 @@ using namespace nav;
 using Clock = std::chrono::steady_clock;
 
 constexpr int kServeBlock = 128;
+// The world volume's daughter boxes are staged in LDS by the step kernel
+// when they take at most this many bytes.
+constexpr size_t kServeBoxLds = 8192;
 constexpr int kMaxStreams = 2;
 constexpr int kMaxSegs = 4096;  // requests merged into one launch at most
 
@@ -104,7 +107,20 @@ __global__ __launch_bounds__(kServeBlock) void ServeLocateKernel(const NavDev g,
 template <int D, uint32_t M, bool MX>
 __global__ __launch_bounds__(kServeBlock) void ServeStepKernel(const NavDev g, const Arena a,
                                                                const Seg *__restrict__ segs, int nseg, uint64_t ntiles,
-                                                               double pstep, int kind) {
+                                                               double pstep, int kind, int stage_lo, int stage_hi) {
+  // The world volume's daughter-slot boxes in LDS (as the trace kernel
+  // stages them; none when stage_lo == stage_hi).
+  extern __shared__ float serve_boxes[];
+  NavDev gl = g;
+  if (stage_hi > stage_lo) {  // uniform over the grid
+    const int nf = 8 * (stage_hi - stage_lo);
+    const float *src = g.slot_box + 8 * (size_t)stage_lo;
+    for (int j = threadIdx.x; j < nf; j += kServeBlock) serve_boxes[j] = src[j];
+    __syncthreads();
+    gl.lds_box = serve_boxes;
+    gl.lds_lo = stage_lo;
+    gl.lds_hi = stage_hi;
+  }
   const uint64_t tile = blockIdx.x;
   if (tile >= ntiles) return;
   const Seg sg = segs[FindSeg(segs, nseg, tile)];
@@ -123,7 +139,7 @@ __global__ __launch_bounds__(kServeBlock) void ServeStepKernel(const NavDev g, c
   V3 p{a.x[s], a.y[s], a.z[s]};
   const V3 d{a.dx[s], a.dy[s], a.dz[s]};
   double step;
-  FindNextBoundaryAndStep<D, false, M, MX>(g, p, d, cur, nst, pstep, step, kind);
+  FindNextBoundaryAndStep<D, false, M, MX>(gl, p, d, cur, nst, pstep, step, kind);
   const double sp = step + 1E-6;
   p = V3{p.x + d.x * sp, p.y + d.y * sp, p.z + d.z * sp};
   a.x[s] = p.x;
@@ -146,6 +162,8 @@ using namespace vga;
 struct vg_server {
   const vg_geometry *g = nullptr;
   NavDev dev{};
+  int stage_lo = 0, stage_hi = 0;  // world daughter slots staged in LDS by the step kernel
+  size_t box_bytes = 0;
   Arena a{};
   int navigator = VG_NAV_ABBOX;
   double pstep = 0;
@@ -233,8 +251,8 @@ void StartLaunch(vg_server *s, int k) {
                   "serve segment upload");
     if (!rc) {
       VGA_NAV_VARIANTS_MX(s->g, s->navigator == VG_NAV_ABBOX_MIXED, ServeStepKernel, dim3((unsigned)tiles),
-                          dim3(kServeBlock), 0, s->stream[k], s->dev, s->a, s->d_segs[k], nseg, tiles, s->pstep,
-                          s->navigator);
+                          dim3(kServeBlock), s->box_bytes, s->stream[k], s->dev, s->a, s->d_segs[k], nseg, tiles,
+                          s->pstep, s->navigator, s->stage_lo, s->stage_hi);
       rc = CheckHip(hipGetLastError(), "serve step launch");
     }
   }
@@ -439,6 +457,15 @@ int vg_server_create(const vg_geometry *g, uint64_t hbm_budget_bytes, int32_t na
   s->pstep = pstep;
   s->dev = NavDev{g->d_placed, g->d_params, g->d_daughters, g->d_aabb, g->d_b32, g->d_b32side, g->d_slot_aabb,
                   g->world,    (int32_t)g->placed.size(), (int32_t)g->daughters.size(), 0};
+  if (g->world >= 0) {
+    const PlacedRec &w = g->placed[g->world];
+    const size_t bytes = 8 * sizeof(float) * (size_t)(w.dend - w.dbegin);
+    if (bytes > 0 && bytes <= kServeBoxLds) {
+      s->stage_lo = w.dbegin;
+      s->stage_hi = w.dend;
+      s->box_bytes = bytes;
+    }
+  }
   rc = CheckHip(hipMalloc(&s->mem, cap * SlotBytes(depth)), "hipMalloc(serve arena)");
   if (!rc) {
     char *m = static_cast<char *>(s->mem);
