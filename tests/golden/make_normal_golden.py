@@ -14,7 +14,10 @@ surface tolerances; the Box also gets its edges and corners.  Rays: N interior
 points with isotropic directions.  Outputs come from the strict build
 (-O2 -fno-fast-math), except where the Cone's normal kernel would assert (a
 point on no surface, volumes/kernel/ConeImplementation.h:2112): those come
-from the Release build (NDEBUG), and `from_release` marks them.
+from the Release build (NDEBUG), and `from_release` marks them.  Which points
+those are is decided by the oracle's valid flag (the strict build cannot be
+asked without aborting); tests/test_oracle_normal.py checks that flag against
+the strict build on every other point and the Release build on these.
 """
 import json
 import os
