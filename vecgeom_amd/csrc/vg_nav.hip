@@ -157,6 +157,8 @@ struct TraceArgs {
   int32_t kind;
   int32_t max_steps;
   int32_t stage_lo, stage_hi;  // daughter slots copied to LDS first (none when equal)
+  int32_t stage_bounds;        // 1: their FP32 bound records too (mixed kernel)
+  int32_t pad;
 };
 
 // The world volume's daughter-slot boxes (8 floats each) are staged in LDS
@@ -164,6 +166,9 @@ struct TraceArgs {
 // workgroups per CU (five waves per SIMD, two waves a workgroup) then still
 // fit in 160 KiB.
 constexpr size_t kNavBoxLds = 8192;
+// ... and, in the mixed kernel, their FP32 bound records (144 bytes each)
+// when those take at most this many bytes more.
+constexpr size_t kNavBoundLds = 9216;
 
 // Occupancy floor of the fused trace kernel over the basic shape set: it is
 // latency-bound, and 5 waves per SIMD (instead of the 3 its 145 VGPRs allow,
@@ -175,7 +180,7 @@ constexpr int TraceWaves(uint32_t m) { return m == kBasicShapes ? 5 : 1; }
 template <int D, uint32_t M, bool MX>
 __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(TraceWaves(M)))) void TraceKernel(
     const NavDev g, const TraceArgs a) {
-  extern __shared__ float nav_boxes[];
+  extern __shared__ __align__(16) float nav_boxes[];
   NavDev gl = g;
   if (a.stage_hi > a.stage_lo) {  // uniform over the grid
     const int nf = 8 * (a.stage_hi - a.stage_lo);
@@ -185,6 +190,20 @@ __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(Trace
     gl.lds_box = nav_boxes;
     gl.lds_lo = a.stage_lo;
     gl.lds_hi = a.stage_hi;
+    if constexpr (MX) {
+      if (a.stage_bounds) {  // uniform over the grid
+        // Records follow the boxes (8 floats a slot keep them 16-byte aligned).
+        constexpr int kRec = (int)(sizeof(B32) / sizeof(uint32_t));
+        uint32_t *dst = reinterpret_cast<uint32_t *>(nav_boxes + nf);
+        const int nr = kRec * (a.stage_hi - a.stage_lo);
+        for (int j = threadIdx.x; j < nr; j += kNavBlock) {
+          const int slot = a.stage_lo + j / kRec;
+          dst[j] = reinterpret_cast<const uint32_t *>(g.b32 + g.daughters[slot])[j % kRec];
+        }
+        __syncthreads();
+        gl.lds_bound = reinterpret_cast<const B32 *>(dst);
+      }
+    }
   }
   const uint64_t i = (uint64_t)blockIdx.x * kNavBlock + threadIdx.x;
   if (i >= a.n) return;
@@ -254,12 +273,19 @@ int NavTrace(const vg_geometry *g, int kind, const double *x, const double *y, c
   TraceArgs a{x, y, z, dx, dy, dz, nsteps, fx, fy, fz, travelled, hash, exact, n, pstep, kind, max_steps, 0, 0};
   const PlacedRec *w = g->world >= 0 ? &g->placed[g->world] : nullptr;
   const size_t box_bytes = w ? 8 * sizeof(float) * (size_t)(w->dend - w->dbegin) : 0;
+  size_t lds = 0;
   if (box_bytes > 0 && box_bytes <= kNavBoxLds) {
     a.stage_lo = w->dbegin;
     a.stage_hi = w->dend;
+    lds = box_bytes;
+    const size_t bound_bytes = sizeof(B32) * (size_t)(w->dend - w->dbegin);
+    if (kind == VG_NAV_ABBOX_MIXED && g->d_b32 && bound_bytes <= kNavBoundLds) {
+      a.stage_bounds = 1;
+      lds += bound_bytes;
+    }
   }
-  VGA_NAV_VARIANTS_MX(g, kind == VG_NAV_ABBOX_MIXED, TraceKernel, dim3(Blocks(n)), dim3(kNavBlock),
-                      a.stage_hi > a.stage_lo ? box_bytes : 0, st, Dev(g), a);
+  VGA_NAV_VARIANTS_MX(g, kind == VG_NAV_ABBOX_MIXED, TraceKernel, dim3(Blocks(n)), dim3(kNavBlock), lds, st, Dev(g),
+                      a);
   return CheckHip(hipGetLastError(), "trace launch");
 }
 

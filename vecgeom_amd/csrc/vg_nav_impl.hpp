@@ -49,6 +49,9 @@ struct NavDev {
   const float *lds_box = nullptr;
   int32_t lds_lo = 0;
   int32_t lds_hi = 0;
+  // The FP32 bound records of the same slots' daughters, in slot order
+  // (mixed trace kernel only; null elsewhere).
+  const B32 *lds_bound = nullptr;
   VGA_NAV const PlacedRec &Pv(int id) const {
     VGA_NAV_CHECK(id >= 0 && id < n_placed, "placed id out of range");
     return placed[id];
@@ -69,6 +72,11 @@ struct NavDev {
   VGA_NAV const B32 &Bound(int id) const {
     VGA_NAV_CHECK(id >= 0 && id < n_placed, "bound id out of range");
     return b32[id];
+  }
+  // The bound record of daughter slot k, whose placed id is id.
+  VGA_NAV const B32 &SlotBound(int k, int id) const {
+    if (lds_bound && k >= lds_lo && k < lds_hi) return lds_bound[k - lds_lo];
+    return Bound(id);
   }
 };
 
@@ -499,7 +507,7 @@ VGA_NAV int AbboxCandidates(const NavDev &g, int db, int de, V3 lp, V3 ld, doubl
     const int cand = g.Daughter(k);
     if (Mixed) {
       float so, si, m;
-      Bound32<M>(g.Bound(cand), g.b32side, lp32, so, si, m);
+      Bound32<M>(g.SlotBound(k, cand), g.b32side, lp32, so, si, m);
       if ((double)so - (double)m > Max(best, 0.) + 1e-6) return true;
     }
     const PlacedRec &r = g.Pv(cand);
