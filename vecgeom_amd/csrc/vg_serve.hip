@@ -51,6 +51,9 @@ constexpr int kServeBlock = 128;
 // The world volume's daughter boxes are staged in LDS by the step kernel
 // when they take at most this many bytes.
 constexpr size_t kServeBoxLds = 8192;
+// ... and, with the mixed navigator, their FP32 bound records (144 bytes
+// each) when those take at most this many bytes more.
+constexpr size_t kServeBoundLds = 9216;
 constexpr int kMaxStreams = 2;
 constexpr int kMaxSegs = 4096;  // requests merged into one launch at most
 
@@ -107,10 +110,12 @@ __global__ __launch_bounds__(kServeBlock) void ServeLocateKernel(const NavDev g,
 template <int D, uint32_t M, bool MX>
 __global__ __launch_bounds__(kServeBlock) void ServeStepKernel(const NavDev g, const Arena a,
                                                                const Seg *__restrict__ segs, int nseg, uint64_t ntiles,
-                                                               double pstep, int kind, int stage_lo, int stage_hi) {
+                                                               double pstep, int kind, int stage_lo, int stage_hi,
+                                                               int stage_bounds) {
   // The world volume's daughter-slot boxes in LDS (as the trace kernel
-  // stages them; none when stage_lo == stage_hi).
-  extern __shared__ float serve_boxes[];
+  // stages them; none when stage_lo == stage_hi), and in the mixed kernel
+  // their FP32 bound records after them.
+  extern __shared__ __align__(16) float serve_boxes[];
   NavDev gl = g;
   if (stage_hi > stage_lo) {  // uniform over the grid
     const int nf = 8 * (stage_hi - stage_lo);
@@ -120,6 +125,19 @@ __global__ __launch_bounds__(kServeBlock) void ServeStepKernel(const NavDev g, c
     gl.lds_box = serve_boxes;
     gl.lds_lo = stage_lo;
     gl.lds_hi = stage_hi;
+    if constexpr (MX) {
+      if (stage_bounds) {  // uniform over the grid: the same daughters' FP32 bound records
+        constexpr int kRec = (int)(sizeof(B32) / sizeof(uint32_t));
+        uint32_t *dst = reinterpret_cast<uint32_t *>(serve_boxes + nf);
+        const int nr = kRec * (stage_hi - stage_lo);
+        for (int j = threadIdx.x; j < nr; j += kServeBlock) {
+          const int slot = stage_lo + j / kRec;
+          dst[j] = reinterpret_cast<const uint32_t *>(g.b32 + g.daughters[slot])[j % kRec];
+        }
+        __syncthreads();
+        gl.lds_bound = reinterpret_cast<const B32 *>(dst);
+      }
+    }
   }
   const uint64_t tile = blockIdx.x;
   if (tile >= ntiles) return;
@@ -163,7 +181,8 @@ struct vg_server {
   const vg_geometry *g = nullptr;
   NavDev dev{};
   int stage_lo = 0, stage_hi = 0;  // world daughter slots staged in LDS by the step kernel
-  size_t box_bytes = 0;
+  int stage_bounds = 0;            // 1: their FP32 bound records too (mixed navigator)
+  size_t box_bytes = 0;            // LDS bytes of the step kernel
   Arena a{};
   int navigator = VG_NAV_ABBOX;
   double pstep = 0;
@@ -252,7 +271,7 @@ void StartLaunch(vg_server *s, int k) {
     if (!rc) {
       VGA_NAV_VARIANTS_MX(s->g, s->navigator == VG_NAV_ABBOX_MIXED, ServeStepKernel, dim3((unsigned)tiles),
                           dim3(kServeBlock), s->box_bytes, s->stream[k], s->dev, s->a, s->d_segs[k], nseg, tiles,
-                          s->pstep, s->navigator, s->stage_lo, s->stage_hi);
+                          s->pstep, s->navigator, s->stage_lo, s->stage_hi, s->stage_bounds);
       rc = CheckHip(hipGetLastError(), "serve step launch");
     }
   }
@@ -464,6 +483,11 @@ int vg_server_create(const vg_geometry *g, uint64_t hbm_budget_bytes, int32_t na
       s->stage_lo = w.dbegin;
       s->stage_hi = w.dend;
       s->box_bytes = bytes;
+      const size_t bound_bytes = sizeof(B32) * (size_t)(w.dend - w.dbegin);
+      if (navigator == VG_NAV_ABBOX_MIXED && g->d_b32 && bound_bytes <= kServeBoundLds) {
+        s->stage_bounds = 1;
+        s->box_bytes += bound_bytes;
+      }
     }
   }
   rc = CheckHip(hipMalloc(&s->mem, cap * SlotBytes(depth)), "hipMalloc(serve arena)");
