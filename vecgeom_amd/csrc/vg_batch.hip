@@ -1144,8 +1144,18 @@ int vg_batch_autotune(vg_batch *b, int32_t repetitions, void *stream, float *bes
   if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   // Small persistent grids leave wave slots to the other classes, which then
   // run beside the light class (it is bound by HBM, they by latency).
-  const unsigned grids[7] = {0u, (unsigned)cus, (unsigned)cus * 2, (unsigned)cus * 3, (unsigned)cus * 4,
-                             (unsigned)cus * 8, (unsigned)cus * 16};
+  const unsigned grids[12] = {0u,
+                              (unsigned)cus,
+                              (unsigned)cus * 9 / 8,
+                              (unsigned)cus * 5 / 4,
+                              (unsigned)cus * 11 / 8,
+                              (unsigned)cus * 3 / 2,
+                              (unsigned)cus * 7 / 4,
+                              (unsigned)cus * 2,
+                              (unsigned)cus * 3,
+                              (unsigned)cus * 4,
+                              (unsigned)cus * 8,
+                              (unsigned)cus * 16};
   hipEvent_t e0, e1;
   if ((rc = CheckHip(hipEventCreate(&e0), "hipEventCreate"))) return rc;
   if ((rc = CheckHip(hipEventCreate(&e1), "hipEventCreate"))) {
