@@ -122,6 +122,10 @@ def parse():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage batched form")
     ap.add_argument("--no-boolean", action="store_true", help="skip the Boolean-volume queries")
     ap.add_argument("--no-normal", action="store_true", help="skip the surface-normal queries")
+    ap.add_argument("--batch-launch", default=None, metavar="IT,BLOCKS",
+                    help="launch shape of the FP64 batched form instead of autotuning it (items per lane, "
+                         "persistent workgroups or 0 for one per tile): profiling runs use the shape the "
+                         "autotuner picked in the timed run")
     ap.add_argument("--nav-tracks", type=int, default=1 << 20, help="tracks of the nested-geometry navigation run")
     ap.add_argument("--nav-steps", type=int, default=3, help="timed trace launches")
     ap.add_argument("--no-nav", action="store_true")
@@ -662,7 +666,12 @@ def batch_forms(args, wl, stream):
 
     res = {}
     bt = B.Batch(wl.geom, wl.tasks())
-    best_ms = bt.autotune(3)
+    if args.batch_launch:
+        it_, blocks_ = (int(v) for v in args.batch_launch.split(","))
+        bt.set_launch(it_, blocks_)
+        best_ms = None
+    else:
+        best_ms = bt.autotune(3)
     it, blocks, items = bt.launch()
     # Timed region: HIP events around each class kernel on the stream it runs
     # on (vg_batch_set_timing); a handful of events per step.
@@ -671,7 +680,7 @@ def batch_forms(args, wl, stream):
     classes = [c for c in bt.class_stats() if c["runs"] > 0]
     bt.set_timing(False)
     res = {"value": round(wl.queries_per_step * args.steps / el / 1e9, 3), "ms_per_step": round(1e3 * el / args.steps, 4),
-           "items_per_lane": it, "blocks": blocks, "autotune_ms": round(best_ms, 4),
+           "items_per_lane": it, "blocks": blocks, "autotune_ms": None if best_ms is None else round(best_ms, 4),
            "achieved_gbs": round(wl.bytes_per_step / (el / args.steps) / 1e9, 1)}
     for c in classes:
         c["achieved_gbs"] = round(c["bytes"] / (c["mean_ms"] * 1e-3) / 1e9, 1) if c["mean_ms"] > 0 else None
